@@ -1,0 +1,79 @@
+// Shared definitions of the engine's kernels and host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdpd {
+
+constexpr int kMaxRep = 1024;        // replicas per context
+constexpr int kWave = 64;            // CDNA wavefront
+constexpr uint32_t kCodeShift = 26;  // pair-list entry: j | stencil code << 26 (2 bits per axis)
+constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
+
+// error flags (device word, sticky until read)
+enum : uint32_t {
+  FLAG_NLIST_OVERFLOW = 1u,
+  FLAG_NONFINITE = 2u,
+  FLAG_OUT_OF_BOX = 4u,
+};
+
+// Per-replica constants, device resident.  FP32 copies of the pair
+// coefficients feed the pair passes; FP64 values feed integration and the
+// guard-band re-decisions.
+struct Replica {
+  double box[3];
+  double inv_box[3];
+  double h[3];          // cell edge = box / nc
+  double inv_h[3];
+  double dt;
+  double rd2;           // density cutoff^2 (FP64 decisions)
+  double conv_p;
+  double cf_s, ibond;   // bond spring, 1/bond_leng
+  int32_t nc[3];
+  uint32_t cell_off;    // first global cell of this replica
+  uint32_t bead_off;    // first bead slot
+  uint32_t n_beads;
+  uint32_t n_cells;
+  uint32_t pair_seed;
+  uint32_t chem_seed;
+  int32_t conv_from, conv_to;
+  int32_t n_species, n_dens, envelope;
+  float hf[3];
+  float rd, inv_rd, crho, inv_rdm, arc;
+  float rd2f;
+  int32_t rdm_is_one;
+  float a0[3][3], c1[3][3], sigma[3][3], gamma[3][3];
+  float mb[3][3][3];
+};
+
+// Guard band on r^2 (FP32 local-coordinate differences carry |error| of a
+// few 1e-7); inside it the pair decision is redone in FP64 from the master
+// positions with the oracle's minimum-image expression.
+constexpr float kGuard = 2.0e-5f;
+
+__host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// replica of bead slot i (bead_off ascending, n_rep entries + sentinel)
+__device__ __forceinline__ int find_rep(const uint32_t* __restrict__ off, int n_rep, uint32_t i) {
+  int lo = 0, hi = n_rep - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// XCD-aware block order: the dispatcher hands consecutive blocks to the 8
+// XCDs round robin; this gives each XCD a contiguous run of tiles so that
+// neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  constexpr uint32_t X = 8;
+  const uint32_t per = nb / X, rem = nb % X;
+  const uint32_t x = b % X, k = b / X;
+  // XCDs 0..rem-1 hold per+1 tiles, the others per
+  const uint32_t base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+  const uint32_t cnt = x < rem ? per + 1 : per;
+  return (k < cnt) ? base + k : b;  // tail (only when nb % X != 0 spills) keeps identity
+}
+
+}  // namespace mdpd

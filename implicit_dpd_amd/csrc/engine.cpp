@@ -1,0 +1,933 @@
+// Host side of the engine: context, device memory, step orchestration,
+// hipGraph capture and the C ABI of include/mdpd_capi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace mdpd;
+
+namespace mdpd {
+thread_local std::string g_err;
+const char* kPhaseNames[kPhases] = {"drift", "sort", "gather", "density", "force",
+                                    "bonded", "kick", "cm_drift", "conversion", "clock"};
+
+int fail(mdpd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_err = msg;
+  return code;
+}
+
+void invalidate_graphs(mdpd_ctx* c) {
+  for (auto& g : c->gexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+}  // namespace mdpd
+
+namespace {
+
+template <class T>
+int dalloc(mdpd_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  HIPCHK(c, hipMalloc(&q, bytes));
+  HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->st));
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return MDPD_OK;
+}
+
+int ensure_pinned(mdpd_ctx* c, size_t bytes) {
+  if (bytes <= c->pinned_bytes) return MDPD_OK;
+  if (c->pinned) hipHostFree(c->pinned);
+  c->pinned = nullptr;
+  c->pinned_bytes = 0;
+  HIPCHK(c, hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
+  c->pinned_bytes = bytes;
+  return MDPD_OK;
+}
+
+int check_params(const mdpd_params& p, std::string& why) {
+  for (int a = 0; a < 3; ++a)
+    if (!(p.box[a] >= 3.0) || !std::isfinite(p.box[a])) {
+      why = "box edges must be finite and >= 3 cutoffs";
+      return MDPD_ERR_ARG;
+    }
+  if (!(p.dt > 0.0)) { why = "dt must be > 0"; return MDPD_ERR_ARG; }
+  if (p.n_species < 1 || p.n_species > 3) { why = "n_species must be 1..3"; return MDPD_ERR_ARG; }
+  if (p.n_dens != 1 && p.n_dens != 3) { why = "n_dens must be 1 or 3"; return MDPD_ERR_ARG; }
+  if (p.envelope != 0 && p.envelope != 1) { why = "envelope must be 0 or 1"; return MDPD_ERR_ARG; }
+  if (!(p.rd > 0.0 && p.rd <= 1.0)) { why = "rd must be in (0, 1]"; return MDPD_ERR_ARG; }
+  if (!(p.rdm > 0.0 && p.rdm <= 1.0)) { why = "rdm must be in (0, 1]"; return MDPD_ERR_ARG; }
+  if (p.conv_p > 0.0 && (p.conv_from < 0 || p.conv_from > 2 || p.conv_to < 0 || p.conv_to > 2)) {
+    why = "conv_from/conv_to must be bead types 0..2";
+    return MDPD_ERR_ARG;
+  }
+  return MDPD_OK;
+}
+
+Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off, uint32_t nb) {
+  Replica r{};
+  for (int a = 0; a < 3; ++a) {
+    r.box[a] = p.box[a];
+    r.inv_box[a] = 1.0 / p.box[a];
+    r.nc[a] = (int)std::floor(p.box[a] / 1.0);
+    r.h[a] = p.box[a] / r.nc[a];
+    r.inv_h[a] = 1.0 / r.h[a];
+    r.hf[a] = (float)r.h[a];
+  }
+  r.dt = p.dt;
+  r.rd2 = p.rd * p.rd;
+  r.conv_p = p.conv_p;
+  r.cf_s = p.cf_s;
+  r.ibond = (p.bond_leng != 0.0) ? 1.0 / p.bond_leng : 0.0;
+  r.cell_off = cell_off;
+  r.bead_off = bead_off;
+  r.n_beads = nb;
+  r.n_cells = (uint32_t)r.nc[0] * r.nc[1] * r.nc[2];
+  r.pair_seed = p.pair_seed;
+  r.chem_seed = p.chem_seed;
+  r.conv_from = p.conv_from;
+  r.conv_to = p.conv_to;
+  r.n_species = p.n_species;
+  r.n_dens = p.n_dens;
+  r.envelope = p.envelope;
+  r.rd = (float)p.rd;
+  r.inv_rd = (float)(1.0 / p.rd);
+  r.crho = (float)p.crho;
+  r.inv_rdm = (float)(1.0 / p.rdm);
+  r.arc = (float)p.arc;
+  r.rd2f = (float)(p.rd * p.rd);
+  r.rdm_is_one = (p.rdm == 1.0) ? 1 : 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      r.a0[i][j] = (float)p.a0[i][j];
+      r.c1[i][j] = (float)p.c1[i][j];
+      r.sigma[i][j] = (float)p.sigma[i][j];
+      r.gamma[i][j] = (float)p.gamma[i][j];
+      for (int k = 0; k < 3; ++k) r.mb[i][j][k] = (float)p.mb[i][j][k];
+    }
+  return r;
+}
+
+int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t capacity,
+                  mdpd_ctx** out) {
+  if (!p || !out || n_rep < 1 || n_rep > kMaxRep) return fail(nullptr, MDPD_ERR_ARG, "bad arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, MDPD_ERR_NO_DEVICE, "no HIP device");
+  std::string why;
+  for (int r = 0; r < n_rep; ++r) {
+    int rc = check_params(p[r], why);
+    if (rc) return fail(nullptr, rc, "replica " + std::to_string(r) + ": " + why);
+    if (p[r].n_dens != p[0].n_dens || p[r].envelope != p[0].envelope)
+      return fail(nullptr, MDPD_ERR_ARG, "replicas must share n_dens and envelope");
+  }
+  mdpd_ctx* c = new mdpd_ctx();
+  hipGetDevice(&c->dev);
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, MDPD_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->n_rep = n_rep;
+  c->P.assign(p, p + n_rep);
+  c->nd = p[0].n_dens;
+  c->env = p[0].envelope;
+  c->ns = 1;
+  c->cm_corr = p[0].cm_corr ? p[0].cm_corr : 100u;
+  for (int r = 0; r < n_rep; ++r) {
+    if (p[r].n_species > 1) c->ns = 3;
+    if (p[r].conv_p > 0.0) c->any_conv = true;
+  }
+  c->rep_off.assign(n_rep + 1, 0);
+  c->cell_off.assign(n_rep + 1, 0);
+  uint32_t tot = 0, cells = 0;
+  for (int r = 0; r < n_rep; ++r) {
+    c->rep_off[r] = tot;
+    c->cell_off[r] = cells;
+    const uint32_t nbr = nb ? nb[r] : capacity;
+    Replica rr = make_replica(p[r], cells, tot, nbr);
+    c->R.push_back(rr);
+    tot += nbr;
+    cells += rr.n_cells;
+  }
+  c->rep_off[n_rep] = tot;
+  c->cell_off[n_rep] = cells;
+  c->cap = nb ? tot : capacity;
+  if (c->cap == 0 || c->cap > kIndexMask) {
+    mdpd_destroy(c);
+    return fail(nullptr, MDPD_ERR_ARG, "capacity must be in 1.." + std::to_string(kIndexMask));
+  }
+  c->ncell = cells;
+  c->amp_cap = c->cap / 4 + 1;
+  c->amp_off.assign(n_rep + 1, 0);
+  c->amp_num_rep.assign(n_rep, 0);
+  c->sol_num_rep.assign(n_rep, 0);
+  c->tune.block_scan = 128;
+  c->tune.block_force = 128;
+  c->tune.cell_div = 1;
+  c->tune.nlist_width = 96;
+  c->tune.reorder_every = 0;
+  c->tune.use_graph = 1;
+
+  DevState& d = c->d;
+  const uint32_t cap = c->cap;
+  const uint32_t cap64 = div_up(cap, 64) * 64;
+  int rc = MDPD_OK;
+#define ALLOC(ptr, cnt) \
+  if ((rc = dalloc(c, &(ptr), (cnt))) != MDPD_OK) { g_err = c->err; mdpd_destroy(c); return rc; }
+  ALLOC(d.pos, 3 * (size_t)cap);
+  ALLOC(d.vel, 3 * (size_t)cap);
+  ALLOC(d.vbuf, 3 * (size_t)cap);
+  ALLOC(d.delta, 3 * (size_t)cap);
+  ALLOC(d.acc, 3 * (size_t)cap);
+  ALLOC(d.press, 3 * (size_t)cap);
+  ALLOC(d.id, cap);
+  ALLOC(d.prop, cap);
+  ALLOC(d.amp_id, cap);
+  ALLOC(d.unit, cap);
+  ALLOC(d.key, cap);
+  ALLOC(d.cell_cnt, (size_t)cells + 1);
+  ALLOC(d.cell_start, (size_t)cells + 1);
+  ALLOC(d.cursor, (size_t)cells + 1);
+  ALLOC(d.bsum, (size_t)div_up(cells, 1024) + 1);
+  ALLOC(d.perm, cap64);
+  ALLOC(d.spos, cap64);
+  ALLOC(d.svel, cap64);
+  ALLOC(d.scell, cap64);
+  ALLOC(d.sdens, (size_t)c->nd * cap64);
+  ALLOC(d.nlist, (size_t)cap64 * c->tune.nlist_width);
+  ALLOC(d.ncount, cap64);
+  ALLOC(d.reps, n_rep);
+  ALLOC(d.rep_off, n_rep + 1);
+  ALLOC(d.rep_cell_off, n_rep + 1);
+  ALLOC(d.topo, 4 * (size_t)c->amp_cap);
+  ALLOC(d.cf_bend, c->amp_cap);
+  ALLOC(d.amp_rep, c->amp_cap);
+  ALLOC(d.amp_off, n_rep + 1);
+  ALLOC(d.d_time, 1);
+  ALLOC(d.flags, 1);
+  ALLOC(d.red, (size_t)n_rep * 64 * 32 + 3 * (size_t)c->amp_cap);
+  ALLOC(d.red_out, (size_t)n_rep * 16);
+  ALLOC(c->vir_bonded, (size_t)n_rep * 3);
+  ALLOC(c->obs_out, (size_t)n_rep * 16);
+  ALLOC(c->d_amp_num, n_rep);
+#undef ALLOC
+  d.cap = cap;
+  d.n = 0;
+  d.ncell = cells;
+  d.W = c->tune.nlist_width;
+  d.n_rep = n_rep;
+  d.n_dens = c->nd;
+  d.n_amp = 0;
+  if (hipMemcpyAsync(d.reps, c->R.data(), sizeof(Replica) * n_rep, hipMemcpyHostToDevice, c->st) !=
+          hipSuccess ||
+      hipMemcpyAsync(d.rep_off, c->rep_off.data(), sizeof(uint32_t) * (n_rep + 1),
+                     hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemcpyAsync(d.rep_cell_off, c->cell_off.data(), sizeof(uint32_t) * (n_rep + 1),
+                     hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
+    mdpd_destroy(c);
+    return fail(nullptr, MDPD_ERR_HIP, "initial upload failed");
+  }
+  for (auto& e : c->ev) hipEventCreate(&e);
+  *out = c;
+  return MDPD_OK;
+}
+
+// one step of main.cpp:167-196 on the context stream
+void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
+  const DevState& d = c->d;
+  hipStream_t st = c->st;
+  const bool prof = c->profiling;
+  auto mark = [&](int k) {
+    if (prof) hipEventRecord(c->ev[k], st);
+  };
+  mark(0);
+  launch_fill_u32(d.cell_cnt, d.ncell + 1, 0u, st);
+  launch_drift(d, true, false, st);
+  mark(1);
+  launch_scan_cells(d, st);
+  launch_scatter(d, st);
+  launch_cell_sort(d, st);
+  mark(2);
+  launch_gather(d, d.vbuf, st);
+  mark(3);
+  launch_density(d, c->ns, c->nd, c->tune.block_scan, st);
+  mark(4);
+  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st);
+  mark(5);
+  launch_bonded(d, virial, c->vir_bonded, st);
+  mark(6);
+  launch_kick(d, st);
+  mark(7);
+  if (cm) launch_cm_drift(d, st);
+  mark(8);
+  if (c->any_conv) launch_head_conversion(d, st);
+  mark(9);
+  launch_time_inc(d.d_time, st);
+  mark(10);
+}
+
+}  // namespace
+
+int mdpd::check_flags(mdpd_ctx* c) {
+  uint32_t f = 0;
+  HIPCHK(c, hipMemcpyAsync(&f, c->d.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (f) {
+    launch_fill_u32(c->d.flags, 1, 0u, c->st);
+    hipStreamSynchronize(c->st);
+    if (f & FLAG_NLIST_OVERFLOW)
+      return fail(c, MDPD_ERR_NLIST_OVERFLOW,
+                  "a bead has more neighbours than nlist_width = " + std::to_string(c->d.W));
+    if (f & FLAG_NONFINITE) return fail(c, MDPD_ERR_NONFINITE, "non-finite position/velocity/force");
+    return fail(c, MDPD_ERR_OUT_OF_BOX, "bead outside the box");
+  }
+  return MDPD_OK;
+}
+
+int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const bool virial = (k + 1 == nsteps);
+    const bool cm = (c->time % c->cm_corr) == 0;
+    if (c->tune.use_graph && !c->profiling) {
+      const int v = (virial ? 1 : 0) | (cm ? 2 : 0);
+      if (!c->gexec[v]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c, virial, cm);
+        HIPCHK(c, hipStreamEndCapture(c->st, &g));
+        HIPCHK(c, hipGraphInstantiate(&c->gexec[v], g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+      }
+      HIPCHK(c, hipGraphLaunch(c->gexec[v], c->st));
+    } else {
+      enqueue_step(c, virial, cm);
+      if (c->profiling) {
+        HIPCHK(c, hipEventSynchronize(c->ev[kPhases]));
+        for (int p = 0; p < kPhases; ++p) {
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, c->ev[p], c->ev[p + 1]);
+          c->ph_ms[p] += ms;
+        }
+        c->ph_steps++;
+      }
+    }
+    c->time++;
+  }
+  c->drifted = false;
+  c->have_cells = true;
+  c->cells_fresh = true;
+  c->have_density = true;
+  return MDPD_OK;
+}
+
+namespace {
+void dens_to_master(mdpd_ctx* c, std::vector<double>& out /*[n][3]*/) {
+  const uint32_t n = c->d.n;
+  out.assign(3 * (size_t)n, 0.0);
+  if (!c->have_density || !n) return;
+  std::vector<uint32_t> perm(n);
+  std::vector<float> sd((size_t)c->nd * n);
+  hipMemcpyAsync(perm.data(), c->d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->st);
+  for (int k = 0; k < c->nd; ++k)
+    hipMemcpyAsync(sd.data() + (size_t)k * n, c->d.sdens + (size_t)k * c->d.cap, sizeof(float) * n,
+                   hipMemcpyDeviceToHost, c->st);
+  hipStreamSynchronize(c->st);
+  for (uint32_t s = 0; s < n; ++s)
+    for (int k = 0; k < c->nd; ++k) out[3 * (size_t)perm[s] + k] = sd[(size_t)k * n + s];
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+const char* mdpd_global_error(void) { return g_err.c_str(); }
+const char* mdpd_last_error(const mdpd_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int mdpd_device_name(char* buf, size_t len) {
+  hipDeviceProp_t pr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&pr, dev) != hipSuccess)
+    return fail(nullptr, MDPD_ERR_NO_DEVICE, "no HIP device");
+  snprintf(buf, len, "%s (%s, %d CUs)", pr.name, pr.gcnArchName, pr.multiProcessorCount);
+  return MDPD_OK;
+}
+
+int mdpd_create(const mdpd_params* p, uint32_t capacity, mdpd_ctx** out) {
+  return create_common(p, nullptr, 1, capacity, out);
+}
+
+int mdpd_create_ensemble(const mdpd_params* p, const uint32_t* n_beads, int32_t n_rep,
+                         mdpd_ctx** out) {
+  if (!n_beads) return fail(nullptr, MDPD_ERR_ARG, "n_beads is null");
+  return create_common(p, n_beads, n_rep, 0, out);
+}
+
+int mdpd_destroy(mdpd_ctx* c) {
+  if (!c) return MDPD_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  invalidate_graphs(c);
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  for (void* q : c->allocs) hipFree(q);
+  if (c->pinned) hipHostFree(c->pinned);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return MDPD_OK;
+}
+
+int mdpd_num_beads(const mdpd_ctx* c, uint32_t* n) {
+  if (!c || !n) return MDPD_ERR_ARG;
+  *n = c->d.n;
+  return MDPD_OK;
+}
+
+int mdpd_upload_soa(mdpd_ctx* c, uint32_t n, const double* pos, const double* vel,
+                    const double* acc, const uint32_t* id, const uint32_t* prop,
+                    const uint32_t* amp_id, const uint32_t* unit) {
+  if (!c || !pos || !vel || !id || !prop) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (n > c->cap) return fail(c, MDPD_ERR_CAPACITY, "more beads than the capacity");
+  if (c->n_rep > 1 && n != c->cap)
+    return fail(c, MDPD_ERR_ARG, "an ensemble upload must hold every replica's beads");
+  // parameter.hpp:436-460 checks
+  for (uint32_t i = 0; i < n; ++i) {
+    const int r = (int)(std::upper_bound(c->rep_off.begin(), c->rep_off.begin() + c->n_rep, i) -
+                        c->rep_off.begin()) - 1;
+    for (int a = 0; a < 3; ++a) {
+      const double x = pos[3 * (size_t)i + a];
+      if (!std::isfinite(x) || !std::isfinite(vel[3 * (size_t)i + a]))
+        return fail(c, MDPD_ERR_NONFINITE, "non-finite position or velocity at bead " + std::to_string(i));
+      if (!(x <= c->R[r].box[a] && x >= 0.0))
+        return fail(c, MDPD_ERR_OUT_OF_BOX, "There is a particle in outside range (bead " + std::to_string(i) + ")");
+    }
+    if (prop[i] > 2) return fail(c, MDPD_ERR_ARG, "bead type must be 0..2");
+  }
+  const size_t bytes = 3 * sizeof(double) * (size_t)n * 4 + 4 * sizeof(uint32_t) * (size_t)n +
+                       3 * sizeof(float) * (size_t)n;
+  int rc = ensure_pinned(c, bytes + 64);
+  if (rc) return rc;
+  double* hp = (double*)c->pinned;
+  double* hv = hp + 3 * (size_t)n;
+  double* hb = hv + 3 * (size_t)n;
+  double* hd = hb + 3 * (size_t)n;
+  float* ha = (float*)(hd + 3 * (size_t)n);
+  uint32_t* hu = (uint32_t*)(ha + 3 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      hp[(size_t)a * n + i] = pos[3 * (size_t)i + a];
+      hv[(size_t)a * n + i] = vel[3 * (size_t)i + a];
+      hb[(size_t)a * n + i] = vel[3 * (size_t)i + a];
+      hd[(size_t)a * n + i] = 0.0;
+      ha[(size_t)a * n + i] = acc ? (float)acc[3 * (size_t)i + a] : 0.f;
+    }
+  for (uint32_t i = 0; i < n; ++i) {
+    hu[i] = id[i];
+    hu[n + i] = prop[i];
+    hu[2 * (size_t)n + i] = amp_id ? amp_id[i] : MDPD_UNIT_NONE;
+    hu[3 * (size_t)n + i] = unit ? unit[i] : MDPD_UNIT_NONE;
+  }
+  DevState& d = c->d;
+  for (int a = 0; a < 3; ++a) {
+    HIPCHK(c, hipMemcpyAsync(d.pos + (size_t)a * d.cap, hp + (size_t)a * n, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.vel + (size_t)a * d.cap, hv + (size_t)a * n, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.vbuf + (size_t)a * d.cap, hb + (size_t)a * n, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.delta + (size_t)a * d.cap, hd + (size_t)a * n, 8 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.acc + (size_t)a * d.cap, ha + (size_t)a * n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(d.id, hu, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(d.prop, hu + n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(d.amp_id, hu + 2 * (size_t)n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(d.unit, hu + 3 * (size_t)n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  // solvent count per replica (unit == none)
+  for (int r = 0; r < c->n_rep; ++r) {
+    uint32_t s = 0;
+    for (uint32_t i = c->rep_off[r]; i < std::min(n, c->rep_off[r + 1]); ++i)
+      if (!unit || unit[i] == MDPD_UNIT_NONE) ++s;
+    c->sol_num_rep[r] = s;
+  }
+  if (n != d.n) invalidate_graphs(c);
+  d.n = n;
+  if (c->n_rep == 1) {
+    c->rep_off[1] = n;
+    c->R[0].n_beads = n;
+    HIPCHK(c, hipMemcpyAsync(d.rep_off, c->rep_off.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  c->drifted = false;
+  c->have_cells = c->cells_fresh = c->have_density = false;
+  return MDPD_OK;
+}
+
+int mdpd_upload_fpdpd(mdpd_ctx* c, const void* aos, uint32_t n, size_t stride) {
+  if (!c || !aos) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (stride < MDPD_FPDPD_STRIDE) return fail(c, MDPD_ERR_ARG, "stride < 232");
+  std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), vb(3 * (size_t)n), acc(3 * (size_t)n),
+      del(3 * (size_t)n);
+  std::vector<uint32_t> id(n), prop(n), aid(n), unit(n);
+  const char* b = (const char*)aos;
+  for (uint32_t i = 0; i < n; ++i) {
+    const char* r = b + (size_t)i * stride;
+    memcpy(&id[i], r + 0, 4);
+    memcpy(&prop[i], r + 4, 4);
+    memcpy(&aid[i], r + 8, 4);
+    memcpy(&unit[i], r + 12, 4);
+    memcpy(&pos[3 * (size_t)i], r + 16, 24);
+    memcpy(&del[3 * (size_t)i], r + 40, 24);
+    memcpy(&vel[3 * (size_t)i], r + 64, 24);
+    memcpy(&vb[3 * (size_t)i], r + 88, 24);
+    memcpy(&acc[3 * (size_t)i], r + 112, 24);
+  }
+  int rc = mdpd_upload_soa(c, n, pos.data(), vel.data(), acc.data(), id.data(), prop.data(),
+                           aid.data(), unit.data());
+  if (rc) return rc;
+  // vel_buf and delta_sumr of the records
+  DevState& d = c->d;
+  std::vector<double> t(n);
+  for (int a = 0; a < 3; ++a) {
+    for (uint32_t i = 0; i < n; ++i) t[i] = vb[3 * (size_t)i + a];
+    HIPCHK(c, hipMemcpy(d.vbuf + (size_t)a * d.cap, t.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    for (uint32_t i = 0; i < n; ++i) t[i] = del[3 * (size_t)i + a];
+    HIPCHK(c, hipMemcpy(d.delta + (size_t)a * d.cap, t.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+  }
+  return MDPD_OK;
+}
+
+int mdpd_download_soa(mdpd_ctx* c, uint32_t n, double* pos, double* vel, double* vel_buf,
+                      double* acc, double* press, double* dens, double* delta, uint32_t* id,
+                      uint32_t* prop, uint32_t* amp_id, uint32_t* unit) {
+  if (!c) return MDPD_ERR_ARG;
+  if (n != c->d.n) return fail(c, MDPD_ERR_ARG, "n must equal the bead count");
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const DevState& d = c->d;
+  std::vector<double> t(n);
+  std::vector<float> f(n);
+  auto get3d = [&](const double* src, double* dst) -> int {
+    if (!dst) return MDPD_OK;
+    for (int a = 0; a < 3; ++a) {
+      HIPCHK(c, hipMemcpy(t.data(), src + (size_t)a * d.cap, 8 * (size_t)n, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)i + a] = t[i];
+    }
+    return MDPD_OK;
+  };
+  auto get3f = [&](const float* src, double* dst) -> int {
+    if (!dst) return MDPD_OK;
+    for (int a = 0; a < 3; ++a) {
+      HIPCHK(c, hipMemcpy(f.data(), src + (size_t)a * d.cap, 4 * (size_t)n, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)i + a] = f[i];
+    }
+    return MDPD_OK;
+  };
+  int rc;
+  if ((rc = get3d(d.pos, pos))) return rc;
+  if ((rc = get3d(c->drifted ? d.vbuf : d.vel, vel))) return rc;
+  if ((rc = get3d(d.vbuf, vel_buf))) return rc;
+  if ((rc = get3d(d.delta, delta))) return rc;
+  if ((rc = get3f(d.acc, acc))) return rc;
+  if ((rc = get3f(d.press, press))) return rc;
+  if (id) HIPCHK(c, hipMemcpy(id, d.id, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (prop) HIPCHK(c, hipMemcpy(prop, d.prop, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (amp_id) HIPCHK(c, hipMemcpy(amp_id, d.amp_id, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (unit) HIPCHK(c, hipMemcpy(unit, d.unit, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (dens) {
+    std::vector<double> dm;
+    dens_to_master(c, dm);
+    memcpy(dens, dm.data(), sizeof(double) * 3 * (size_t)n);
+  }
+  return MDPD_OK;
+}
+
+int mdpd_download_fpdpd(mdpd_ctx* c, void* aos, uint32_t n, size_t stride) {
+  if (!c || !aos) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (stride < MDPD_FPDPD_STRIDE) return fail(c, MDPD_ERR_ARG, "stride < 232");
+  std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), vb(3 * (size_t)n), acc(3 * (size_t)n),
+      pr(3 * (size_t)n), dn(3 * (size_t)n), del(3 * (size_t)n), cm(8 * (size_t)n, 0.0);
+  std::vector<uint32_t> id(n), prop(n), aid(n), unit(n);
+  int rc = mdpd_download_soa(c, n, pos.data(), vel.data(), vb.data(), acc.data(), pr.data(),
+                             dn.data(), del.data(), id.data(), prop.data(), aid.data(), unit.data());
+  if (rc) return rc;
+  if (c->cells_fresh && n) {
+    // nei_cm_pos (ptcl_class.hpp:53-57) from the positions of the last rebuild
+    uint32_t* idx = nullptr;
+    double* out = nullptr;
+    HIPCHK(c, hipMalloc(&idx, 4 * (size_t)n));
+    HIPCHK(c, hipMalloc(&out, 64 * (size_t)n));
+    std::vector<uint32_t> seq(n);
+    for (uint32_t i = 0; i < n; ++i) seq[i] = i;
+    HIPCHK(c, hipMemcpy(idx, seq.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+    launch_neighbor_cm(c->d, idx, n, out, c->st);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(cm.data(), out, 64 * (size_t)n, hipMemcpyDeviceToHost));
+    hipFree(idx);
+    hipFree(out);
+  }
+  char* b = (char*)aos;
+  for (uint32_t i = 0; i < n; ++i) {
+    char* r = b + (size_t)i * stride;
+    memcpy(r + 0, &id[i], 4);
+    memcpy(r + 4, &prop[i], 4);
+    memcpy(r + 8, &aid[i], 4);
+    memcpy(r + 12, &unit[i], 4);
+    memcpy(r + 16, &pos[3 * (size_t)i], 24);
+    memcpy(r + 40, &del[3 * (size_t)i], 24);
+    memcpy(r + 64, &vel[3 * (size_t)i], 24);
+    memcpy(r + 88, &vb[3 * (size_t)i], 24);
+    memcpy(r + 112, &acc[3 * (size_t)i], 24);
+    memcpy(r + 136, &pr[3 * (size_t)i], 24);
+    memcpy(r + 160, &cm[8 * (size_t)i], 24);
+    memcpy(r + 184, &cm[8 * (size_t)i + 3], 24);
+    memcpy(r + 208, &dn[3 * (size_t)i], 24);
+  }
+  return MDPD_OK;
+}
+
+int mdpd_set_topology(mdpd_ctx* c, const uint32_t* topo, uint32_t n_amp, const uint32_t* core_amp_id,
+                      uint32_t n_core) {
+  if (!c || (n_amp && !topo)) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (n_amp > c->amp_cap) return fail(c, MDPD_ERR_CAPACITY, "too many amphiphiles");
+  std::vector<double> cfb(n_amp);
+  std::vector<uint32_t> arep(n_amp);
+  std::fill(c->amp_num_rep.begin(), c->amp_num_rep.end(), 0u);
+  int last = 0;
+  for (uint32_t a = 0; a < n_amp; ++a) {
+    for (int u = 0; u < 4; ++u)
+      if (topo[4 * (size_t)a + u] >= c->d.n)
+        return fail(c, MDPD_ERR_ARG, "topology index out of range (amphiphile " + std::to_string(a) + ")");
+    const uint32_t b0 = topo[4 * (size_t)a];
+    const int r = (int)(std::upper_bound(c->rep_off.begin(), c->rep_off.begin() + c->n_rep, b0) -
+                        c->rep_off.begin()) - 1;
+    if (r < last) return fail(c, MDPD_ERR_ARG, "chains must be grouped by replica");
+    last = r;
+    arep[a] = (uint32_t)r;
+    c->amp_num_rep[r]++;
+    cfb[a] = c->P[r].cf_b;  // bdf_calculator.hpp:77-86
+    for (uint32_t k = 0; k < n_core; ++k)
+      if (core_amp_id && a == core_amp_id[k]) cfb[a] = c->P[r].cf_b_rigid;
+  }
+  c->amp_off.assign(c->n_rep + 1, 0);
+  for (int r = 0; r < c->n_rep; ++r) c->amp_off[r + 1] = c->amp_off[r] + c->amp_num_rep[r];
+  DevState& d = c->d;
+  if (n_amp) {
+    HIPCHK(c, hipMemcpy(d.topo, topo, 16 * (size_t)n_amp, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.cf_bend, cfb.data(), 8 * (size_t)n_amp, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.amp_rep, arep.data(), 4 * (size_t)n_amp, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(d.amp_off, c->amp_off.data(), 4 * (size_t)(c->n_rep + 1), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_amp_num, c->amp_num_rep.data(), 4 * (size_t)c->n_rep, hipMemcpyHostToDevice));
+  if (n_amp != d.n_amp) invalidate_graphs(c);
+  d.n_amp = n_amp;
+  c->n_amp = n_amp;
+  return MDPD_OK;
+}
+
+int mdpd_set_clock(mdpd_ctx* c, uint32_t time, uint32_t pair_seed) {
+  if (!c) return MDPD_ERR_ARG;
+  c->time = time;
+  if (c->n_rep == 1) {  // an ensemble keeps the seeds of its replicas
+    c->P[0].pair_seed = pair_seed;
+    c->R[0].pair_seed = pair_seed;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d.reps, c->R.data(), sizeof(Replica) * c->n_rep, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDPD_OK;
+}
+
+int mdpd_get_clock(const mdpd_ctx* c, uint32_t* time) {
+  if (!c || !time) return MDPD_ERR_ARG;
+  *time = c->time;
+  return MDPD_OK;
+}
+
+int mdpd_set_tuning(mdpd_ctx* c, const mdpd_tuning* t) {
+  if (!c || !t) return MDPD_ERR_ARG;
+  auto okb = [](int b) { return b == 64 || b == 128 || b == 256; };
+  if (!okb(t->block_scan) || !okb(t->block_force)) return fail(c, MDPD_ERR_ARG, "block sizes must be 64, 128 or 256");
+  if (t->cell_div != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "cell_div must be 1");
+  if (t->nlist_width < 8 || t->nlist_width > 1024) return fail(c, MDPD_ERR_ARG, "nlist_width must be 8..1024");
+  if (t->nlist_width != c->tune.nlist_width) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const uint32_t cap64 = div_up(c->cap, 64) * 64;
+    uint32_t* nl = nullptr;
+    HIPCHK(c, hipMalloc(&nl, sizeof(uint32_t) * (size_t)cap64 * t->nlist_width));
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d.nlist);
+    if (it != c->allocs.end()) {
+      hipFree(*it);
+      *it = nl;
+    }
+    c->d.nlist = nl;
+    c->d.W = t->nlist_width;
+    c->have_density = false;
+  }
+  c->tune = *t;
+  invalidate_graphs(c);
+  return MDPD_OK;
+}
+
+int mdpd_get_tuning(const mdpd_ctx* c, mdpd_tuning* t) {
+  if (!c || !t) return MDPD_ERR_ARG;
+  *t = c->tune;
+  return MDPD_OK;
+}
+
+int mdpd_drift_predict(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  launch_drift(c->d, false, false, c->st);
+  c->drifted = true;
+  c->cells_fresh = false;
+  return MDPD_OK;
+}
+
+int mdpd_rebuild_cells(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  const DevState& d = c->d;
+  launch_fill_u32(d.cell_cnt, d.ncell + 1, 0u, c->st);
+  launch_keys(d, c->st);
+  launch_scan_cells(d, c->st);
+  launch_scatter(d, c->st);
+  launch_cell_sort(d, c->st);
+  // the force uses the reference's vel: the predicted velocity after a drift
+  launch_gather(d, c->drifted ? d.vbuf : d.vel, c->st);
+  c->have_cells = c->cells_fresh = true;
+  c->have_density = false;
+  return MDPD_OK;
+}
+
+int mdpd_density(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!c->have_cells) return fail(c, MDPD_ERR_STATE, "density before rebuild_cells");
+  launch_density(c->d, c->ns, c->nd, c->tune.block_scan, c->st);
+  c->have_density = true;
+  return check_flags(c);
+}
+
+int mdpd_force(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!c->have_density) return fail(c, MDPD_ERR_STATE, "force before density");
+  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+  launch_force(c->d, c->ns, c->nd, c->env, true, c->tune.block_force, c->st);
+  return check_flags(c);
+}
+
+int mdpd_bonded(mdpd_ctx* c, double vir[3]) {
+  if (!c) return MDPD_ERR_ARG;
+  launch_bonded(c->d, true, c->vir_bonded, c->st);
+  if (vir) {
+    HIPCHK(c, hipMemcpyAsync(vir, c->vir_bonded, 24, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  return MDPD_OK;
+}
+
+int mdpd_kick(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  launch_kick(c->d, c->st);
+  c->drifted = false;
+  return MDPD_OK;
+}
+
+int mdpd_remove_cm_drift(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  if (c->drifted) {
+    // the reference's vel is the predicted velocity here; vel_buf stays
+    for (int a = 0; a < 3; ++a)
+      launch_copy_d(c->d.vel + (size_t)a * c->d.cap, c->d.vbuf + (size_t)a * c->d.cap, c->d.n, c->st);
+    c->drifted = false;
+  }
+  launch_cm_drift(c->d, c->st);
+  return MDPD_OK;
+}
+
+int mdpd_head_conversion(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+  launch_head_conversion(c->d, c->st);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDPD_OK;
+}
+
+int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
+  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+  int rc = run_steps(c, nsteps);
+  if (rc) return rc;
+  return check_flags(c);
+}
+
+int mdpd_sync(mdpd_ctx* c) {
+  if (!c) return MDPD_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDPD_OK;
+}
+
+static int observe_impl(mdpd_ctx* c, mdpd_observables* out, int all) {
+  if (!c || !out) return MDPD_ERR_ARG;
+  const DevState& d = c->d;
+  launch_observe(d, c->d_amp_num, c->obs_out, c->st);
+  std::vector<double> q((size_t)c->n_rep * 12), bv((size_t)c->n_rep * 3);
+  HIPCHK(c, hipMemcpyAsync(q.data(), c->obs_out, sizeof(double) * q.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(bv.data(), c->vir_bonded, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const int nr = all ? c->n_rep : 1;
+  for (int r = 0; r < nr; ++r) {
+    mdpd_observables& o = out[r];
+    memset(&o, 0, sizeof(o));
+    const double* s = &q[(size_t)r * 12];
+    const uint32_t n_r = c->rep_off[r + 1] - c->rep_off[r];
+    for (int a = 0; a < 3; ++a) o.kin_temp[a] = s[a] / n_r;
+    o.kin_temp[3] = (o.kin_temp[0] + o.kin_temp[1] + o.kin_temp[2]) / 3.0;
+    const double iv = (1.0 / c->R[r].box[0]) * (1.0 / c->R[r].box[1]) * (1.0 / c->R[r].box[2]);
+    for (int a = 0; a < 3; ++a) {
+      o.bonded_vir[a] = bv[(size_t)r * 3 + a];
+      o.pressure[a] = (s[3 + a] + o.bonded_vir[a] * 0.5) * iv;
+      o.pair_vir[a] = s[6 + a];
+    }
+    const uint32_t na = c->amp_num_rep[r], ns = c->sol_num_rep[r];
+    o.diffusion[0] = s[9] / n_r;
+    o.diffusion[1] = na ? s[11] / na : 0.0;
+    o.diffusion[2] = ns ? s[10] / ns : 0.0;
+    o.n_beads = n_r;
+    o.n_amp = na;
+    o.time = c->time;
+  }
+  return MDPD_OK;
+}
+
+int mdpd_observe(mdpd_ctx* c, mdpd_observables* out) { return observe_impl(c, out, 0); }
+int mdpd_observe_replicas(mdpd_ctx* c, mdpd_observables* out) { return observe_impl(c, out, 1); }
+
+int mdpd_density_profile(mdpd_ctx* c, int32_t axis, int32_t nbins, double* out) {
+  if (!c || !out || axis < 0 || axis > 2 || nbins < 1) return fail(c, MDPD_ERR_ARG, "bad arguments");
+  uint32_t* h = nullptr;
+  const size_t cnt = (size_t)c->n_rep * nbins;
+  HIPCHK(c, hipMalloc(&h, 4 * cnt));
+  HIPCHK(c, hipMemsetAsync(h, 0, 4 * cnt, c->st));
+  launch_density_profile(c->d, axis, nbins, h, c->st);
+  std::vector<uint32_t> hv(cnt);
+  HIPCHK(c, hipMemcpyAsync(hv.data(), h, 4 * cnt, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  hipFree(h);
+  for (size_t k = 0; k < cnt; ++k) out[k] = hv[k];
+  return MDPD_OK;
+}
+
+int mdpd_neighbor_pairs(mdpd_ctx* c, double cutoff, uint32_t* out_i, uint32_t* out_j, uint64_t cap,
+                        uint64_t* n_out) {
+  if (!c || !n_out) return MDPD_ERR_ARG;
+  if (!c->have_cells || !c->cells_fresh) return fail(c, MDPD_ERR_STATE, "no current cell list");
+  if (!(cutoff > 0.0 && cutoff <= 1.0)) return fail(c, MDPD_ERR_ARG, "cutoff must be in (0, 1]");
+  const uint32_t n = c->d.n;
+  std::vector<uint64_t> keys;
+  if (cutoff == 1.0 && c->have_density) {
+    // the production pair list of the last density pass
+    const uint32_t W = (uint32_t)c->d.W, cap64 = div_up(n, 64) * 64;
+    std::vector<uint32_t> nl((size_t)cap64 * W), cnt(n), perm(n);
+    HIPCHK(c, hipMemcpyAsync(nl.data(), c->d.nlist, 4 * nl.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d.ncount, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(perm.data(), c->d.perm, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (uint32_t s = 0; s < n; ++s) {
+      if (cnt[s] > W) return fail(c, MDPD_ERR_NLIST_OVERFLOW, "pair list overflow");
+      for (uint32_t k = 0; k < cnt[s]; ++k) {
+        const uint32_t e = nl[(size_t)(s >> 6) * W * 64 + (size_t)k * 64 + (s & 63)];
+        keys.push_back(((uint64_t)perm[s] << 32) | perm[e & kIndexMask]);
+      }
+    }
+  } else {
+    unsigned long long* ctr = nullptr;
+    uint32_t *oi = nullptr, *oj = nullptr;
+    const uint64_t lcap = (uint64_t)n * 128 + 1024;
+    HIPCHK(c, hipMalloc(&ctr, 8));
+    HIPCHK(c, hipMalloc(&oi, 4 * lcap));
+    HIPCHK(c, hipMalloc(&oj, 4 * lcap));
+    HIPCHK(c, hipMemsetAsync(ctr, 0, 8, c->st));
+    launch_pairs_list(c->d, cutoff, oi, oj, ctr, lcap, c->st);
+    unsigned long long m = 0;
+    HIPCHK(c, hipMemcpyAsync(&m, ctr, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (m > lcap) return fail(c, MDPD_ERR_CAPACITY, "pair buffer too small");
+    std::vector<uint32_t> hi(m), hj(m);
+    HIPCHK(c, hipMemcpy(hi.data(), oi, 4 * m, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hj.data(), oj, 4 * m, hipMemcpyDeviceToHost));
+    hipFree(ctr);
+    hipFree(oi);
+    hipFree(oj);
+    keys.resize(m);
+    for (uint64_t k = 0; k < m; ++k) keys[k] = ((uint64_t)hi[k] << 32) | hj[k];
+  }
+  std::sort(keys.begin(), keys.end());
+  *n_out = keys.size();
+  if (out_i && out_j)
+    for (uint64_t k = 0; k < keys.size() && k < cap; ++k) {
+      out_i[k] = (uint32_t)(keys[k] >> 32);
+      out_j[k] = (uint32_t)(keys[k] & 0xffffffffu);
+    }
+  return MDPD_OK;
+}
+
+int mdpd_saru_device(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, uint32_t n,
+                     uint32_t* out_u32, float* out_nrml) {
+  if (!s1 || !s2 || !s3 || !out_u32 || !out_nrml) return fail(nullptr, MDPD_ERR_ARG, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MDPD_ERR_NO_DEVICE, "no HIP device");
+  uint32_t *a = nullptr, *b = nullptr, *cc = nullptr, *u = nullptr;
+  float* f = nullptr;
+  const size_t B = 4 * (size_t)std::max<uint32_t>(n, 1);
+  if (hipMalloc(&a, B) || hipMalloc(&b, B) || hipMalloc(&cc, B) || hipMalloc(&u, 8 * B) || hipMalloc(&f, B))
+    return fail(nullptr, MDPD_ERR_HIP, "hipMalloc failed");
+  hipMemcpy(a, s1, 4 * (size_t)n, hipMemcpyHostToDevice);
+  hipMemcpy(b, s2, 4 * (size_t)n, hipMemcpyHostToDevice);
+  hipMemcpy(cc, s3, 4 * (size_t)n, hipMemcpyHostToDevice);
+  launch_saru_test(a, b, cc, n, u, f, nullptr);
+  hipError_t e = hipDeviceSynchronize();
+  hipMemcpy(out_u32, u, 32 * (size_t)n, hipMemcpyDeviceToHost);
+  hipMemcpy(out_nrml, f, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  hipFree(a); hipFree(b); hipFree(cc); hipFree(u); hipFree(f);
+  if (e != hipSuccess) return fail(nullptr, MDPD_ERR_HIP, hipGetErrorString(e));
+  return MDPD_OK;
+}
+
+int mdpd_set_profiling(mdpd_ctx* c, int32_t on) {
+  if (!c) return MDPD_ERR_ARG;
+  c->profiling = on != 0;
+  for (double& x : c->ph_ms) x = 0.0;
+  c->ph_steps = 0;
+  return MDPD_OK;
+}
+
+int mdpd_phase_times(mdpd_ctx* c, double* ms, int32_t cap, int32_t* n, char* names, size_t names_len) {
+  if (!c || !n) return MDPD_ERR_ARG;
+  *n = kPhases;
+  for (int p = 0; p < kPhases && p < cap; ++p) ms[p] = c->ph_steps ? c->ph_ms[p] / c->ph_steps : 0.0;
+  if (names && names_len) {
+    size_t off = 0;
+    for (int p = 0; p < kPhases; ++p) {
+      const size_t l = strlen(kPhaseNames[p]);
+      if (off + l + 1 >= names_len) break;
+      memcpy(names + off, kPhaseNames[p], l);
+      names[off + l] = '\0';
+      off += l + 1;
+    }
+    if (off < names_len) names[off] = '\0';
+  }
+  return MDPD_OK;
+}
+
+}  // extern "C"

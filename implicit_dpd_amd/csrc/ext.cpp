@@ -1,0 +1,265 @@
+// Chemistry event and launch autotuner of the engine (host side).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace mdpd;
+
+namespace {
+
+void min_image_round(const double* box, double d[3]) {
+  for (int a = 0; a < 3; ++a) d[a] -= box[a] * std::round(d[a] * (1.0 / box[a]));
+}
+void normalize(double v[3]) {
+  const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int a = 0; a < 3; ++a) v[a] /= l;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ChemManager::RandomChemEvent, serial (chemmanager.hpp:244-281) with
+// DetermineTargetIdWithCorePos (:116-164) and GetCorePos (:84-114); the
+// cone test and insertion run on the device, the per-core geometry (a few
+// vectors) on the host.
+int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
+  if (!c || !cp || !stop) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (c->n_rep != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
+  if (!c->cells_fresh) return fail(c, MDPD_ERR_STATE, "chemistry needs the cell list of the current positions");
+  if (cp->n_core && !cp->core_ptcl_id) return fail(c, MDPD_ERR_ARG, "core_ptcl_id is null");
+  DevState& d = c->d;
+  const uint32_t nc = cp->n_core;
+  const double* box = c->R[0].box;
+  std::vector<uint32_t> targets;
+  if (nc && d.n_amp) {
+    uint32_t *ids = nullptr, *idx = nullptr;
+    double *cm = nullptr, *core = nullptr;
+    uint8_t* flags = nullptr;
+    HIPCHK(c, hipMalloc(&ids, 4 * nc));
+    HIPCHK(c, hipMalloc(&idx, 4 * nc));
+    HIPCHK(c, hipMalloc(&cm, 64 * nc));
+    HIPCHK(c, hipMalloc(&core, 48 * nc));
+    HIPCHK(c, hipMalloc(&flags, (size_t)nc * d.n_amp));
+    HIPCHK(c, hipMemcpyAsync(ids, cp->core_ptcl_id, 4 * nc, hipMemcpyHostToDevice, c->st));
+    launch_fill_u32(idx, nc, 0xffffffffu, c->st);
+    launch_find_ids(d, ids, nc, idx, c->st);
+    std::vector<uint32_t> hidx(nc);
+    HIPCHK(c, hipMemcpyAsync(hidx.data(), idx, 4 * nc, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<uint32_t> found;
+    for (uint32_t k = 0; k < nc; ++k)
+      if (hidx[k] != 0xffffffffu) found.push_back(hidx[k]);
+    std::vector<double> hcm(8 * (size_t)nc, 0.0), hpos(3 * (size_t)nc, 0.0);
+    if (!found.empty()) {
+      HIPCHK(c, hipMemcpy(idx, found.data(), 4 * found.size(), hipMemcpyHostToDevice));
+      launch_neighbor_cm(d, idx, (uint32_t)found.size(), cm, c->st);
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      HIPCHK(c, hipMemcpy(hcm.data(), cm, 64 * found.size(), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < found.size(); ++k)
+        for (int a = 0; a < 3; ++a)
+          HIPCHK(c, hipMemcpy(&hpos[3 * k + a], d.pos + (size_t)a * d.cap + found[k], 8,
+                              hipMemcpyDeviceToHost));
+    }
+    std::vector<double> hcore(6 * (size_t)nc);
+    size_t fk = 0;
+    for (uint32_t k = 0; k < nc; ++k) {
+      double h[3] = {-1.0, -1.0, -1.0}, t[3] = {-1.0, -1.0, -1.0}, m[3] = {-1.0, -1.0, -1.0};
+      if (hidx[k] != 0xffffffffu) {
+        for (int a = 0; a < 3; ++a) {
+          h[a] = hpos[3 * fk + a];
+          m[a] = hcm[8 * fk + a];
+          t[a] = hcm[8 * fk + 3 + a];
+        }
+        ++fk;
+      }
+      double h2t[3] = {t[0] - h[0], t[1] - h[1], t[2] - h[2]};
+      min_image_round(box, h2t);
+      normalize(h2t);
+      double c2cm[3] = {m[0] - h[0], m[1] - h[1], m[2] - h[2]};
+      min_image_round(box, c2cm);
+      const double prj = c2cm[0] * h2t[0] + c2cm[1] * h2t[1] + c2cm[2] * h2t[2];
+      for (int a = 0; a < 3; ++a) h[a] += prj * h2t[a];
+      for (int a = 0; a < 3; ++a) h[a] -= std::floor(h[a] * (1.0 / box[a])) * box[a];
+      for (int a = 0; a < 3; ++a) {
+        hcore[6 * (size_t)k + a] = h[a];
+        hcore[6 * (size_t)k + 3 + a] = h2t[a];
+      }
+    }
+    HIPCHK(c, hipMemcpy(core, hcore.data(), 48 * (size_t)nc, hipMemcpyHostToDevice));
+    launch_chem_flags(d, core, nc, cp->p_thresld, cp->influ_rad, cp->influ_hei, cp->influ_dep,
+                      cp->influ_grd, c->time, flags, c->st);
+    std::vector<uint8_t> hf((size_t)nc * d.n_amp);
+    HIPCHK(c, hipMemcpyAsync(hf.data(), flags, hf.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<uint32_t> topo(4 * (size_t)d.n_amp);
+    HIPCHK(c, hipMemcpy(topo.data(), d.topo, 16 * (size_t)d.n_amp, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < nc; ++k)
+      for (uint32_t j = 0; j < d.n_amp; ++j)
+        if (hf[(size_t)k * d.n_amp + j]) {
+          targets.push_back(topo[4 * (size_t)j]);
+          targets.push_back(topo[4 * (size_t)j + 2]);
+        }
+    hipFree(ids);
+    hipFree(idx);
+    hipFree(cm);
+    hipFree(core);
+    hipFree(flags);
+  }
+  const uint32_t nt = (uint32_t)(targets.size() / 2);
+  if (nt) {
+    if (d.n + 4 * nt > c->cap || d.n_amp + nt > c->amp_cap)
+      return fail(c, MDPD_ERR_CAPACITY, "inserted amphiphiles exceed the capacity");
+    uint32_t* dt = nullptr;
+    HIPCHK(c, hipMalloc(&dt, 4 * targets.size()));
+    HIPCHK(c, hipMemcpy(dt, targets.data(), 4 * targets.size(), hipMemcpyHostToDevice));
+    launch_chem_insert(d, dt, nt, cp->eps, c->P[0].bond_leng, c->time, c->st);
+    std::vector<double> cfb(nt, c->P[0].cf_b);  // AppendCfBend, bdf_calculator.hpp:21-26
+    HIPCHK(c, hipMemcpyAsync(d.cf_bend + d.n_amp, cfb.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipFree(dt);
+    d.n += 4 * nt;
+    d.n_amp += nt;
+    c->n_amp = d.n_amp;
+    c->amp_num_rep[0] += nt;
+    c->rep_off[1] = d.n;
+    c->R[0].n_beads = d.n;
+    c->amp_off[1] = d.n_amp;
+    HIPCHK(c, hipMemcpy(d.rep_off, c->rep_off.data(), 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.amp_off, c->amp_off.data(), 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_amp_num, c->amp_num_rep.data(), 4, hipMemcpyHostToDevice));
+    invalidate_graphs(c);
+    c->cells_fresh = false;
+    c->have_cells = false;
+    c->have_density = false;
+  }
+  *stop = (c->n_amp >= cp->max_amp_num) ? 1 : 0;
+  return MDPD_OK;
+}
+
+// Times every (block_scan, block_force) pair on the current state for
+// `nsteps` steps and keeps the fastest.  The state is restored after every
+// candidate; per-bead sums run in a fixed order, so every candidate gives
+// the same bits.  With MDPD_TUNE_CACHE set, results are kept in that file.
+int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "autotune between drift and kick");
+  DevState& d = c->d;
+  char keybuf[256];
+  snprintf(keybuf, sizeof(keybuf), "n=%u,ncell=%u,nrep=%d,ns=%d,nd=%d,env=%d,namp=%u,W=%d", d.n,
+           d.ncell, c->n_rep, c->ns, c->nd, c->env, d.n_amp, d.W);
+  const std::string key(keybuf);
+  const char* cache = std::getenv("MDPD_TUNE_CACHE");
+  if (cache) {
+    std::ifstream f(cache);
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream ss(line);
+      std::string k;
+      int bs = 0, bf = 0;
+      if ((ss >> k >> bs >> bf) && k == key) {
+        mdpd_tuning t = c->tune;
+        t.block_scan = bs;
+        t.block_force = bf;
+        int rc = mdpd_set_tuning(c, &t);
+        if (rc) return rc;
+        if (out) *out = c->tune;
+        return MDPD_OK;
+      }
+    }
+  }
+  if (nsteps == 0) nsteps = 1;
+  // snapshot of the evolving state
+  const size_t n3 = 3 * (size_t)d.cap;
+  double* snap = nullptr;
+  float* snapf = nullptr;
+  uint32_t* snapu = nullptr;
+  HIPCHK(c, hipMalloc(&snap, 4 * n3 * sizeof(double)));
+  HIPCHK(c, hipMalloc(&snapf, 2 * n3 * sizeof(float)));
+  HIPCHK(c, hipMalloc(&snapu, (size_t)d.cap * sizeof(uint32_t)));
+  const uint32_t t0 = c->time;
+  auto save = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(snap, d.pos, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + n3, d.vel, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + 2 * n3, d.vbuf, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + 3 * n3, d.delta, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snapf, d.acc, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snapf + n3, d.press, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snapu, d.prop, (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
+    return MDPD_OK;
+  };
+  auto restore = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(d.pos, snap, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.vel, snap + n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.vbuf, snap + 2 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.delta, snap + 3 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.acc, snapf, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.press, snapf + n3, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.prop, snapu, (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
+    c->time = t0;
+    HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MDPD_OK;
+  };
+  int rc = save();
+  if (rc) return rc;
+  const int sizes[3] = {64, 128, 256};
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  float best = 1e30f;
+  mdpd_tuning bt = c->tune;
+  const bool prof = c->profiling;
+  c->profiling = false;
+  for (int bs : sizes)
+    for (int bf : sizes) {
+      mdpd_tuning t = c->tune;
+      t.block_scan = bs;
+      t.block_force = bf;
+      if ((rc = mdpd_set_tuning(c, &t))) break;
+      HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+      if ((rc = run_steps(c, 1))) break;  // warm-up (graph capture)
+      if ((rc = restore())) break;
+      hipEventRecord(e0, c->st);
+      if ((rc = run_steps(c, nsteps))) break;
+      hipEventRecord(e1, c->st);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if ((rc = check_flags(c))) break;
+      if ((rc = restore())) break;
+      if (ms < best) {
+        best = ms;
+        bt = t;
+      }
+    }
+  c->profiling = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(snap);
+  hipFree(snapf);
+  hipFree(snapu);
+  if (rc) return rc;
+  rc = mdpd_set_tuning(c, &bt);
+  if (rc) return rc;
+  c->have_cells = c->cells_fresh = c->have_density = false;
+  if (cache) {
+    std::ofstream f(cache, std::ios::app);
+    f << key << " " << bt.block_scan << " " << bt.block_force << "\n";
+  }
+  if (out) *out = c->tune;
+  return MDPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,305 @@
+// Non-bonded pair passes: the density pass (CalcDensity,
+// nbdf_calculator.hpp:11-61), which also records the pair list, and the
+// force pass (CalcForceEpEpDPD, nbdf_calculator.hpp:63-145).
+//
+// One thread per sorted bead (cell order), gathering over the 27 cells
+// around its own; no Newton-3 and no force atomics: every bead sums its own
+// pairs, as the reference does.  Positions are FP32 offsets inside the
+// bead's cell, so a pair displacement is (l_i - o*h) - l_j with o the
+// stencil offset, accurate to a few 1e-7; pairs whose r^2 lies within
+// kGuard of a cutoff are decided again in FP64 from the master positions
+// with the oracle's minimum-image expression, so the pair sets are those of
+// the FP64 oracle.
+//
+// The density pass writes, per 64-bead tile (one wavefront), a column-major
+// list of the in-range neighbours (sorted slot | stencil code << 26); the
+// force pass walks that list, so the Saru/log/cos/sqrt work of the force
+// runs only on pairs inside the cutoff.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.hpp"
+#include "saru.hpp"
+#include "state.hpp"
+
+namespace mdpd {
+
+__device__ __forceinline__ double fp64_r2(const DevState& d, const Replica& R, uint32_t i,
+                                          uint32_t j) {
+#pragma clang fp contract(off)
+  double x[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double v = d.pos[a * d.cap + i] - d.pos[a * d.cap + j];
+    if (v > 0.5 * R.box[a]) v -= R.box[a];
+    else if (v < -0.5 * R.box[a]) v += R.box[a];
+    x[a] = v;
+  }
+  return x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+}
+
+// Pair decision shared by every pass: returns (r^2 < cut2) as the FP64
+// oracle decides it.  r2 is the FP32 estimate.
+__device__ __forceinline__ bool decide(float r2, float cut2f, double cut2, const DevState& d,
+                                       const Replica& R, uint32_t s, uint32_t t) {
+  if (fabsf(r2 - cut2f) > kGuard) return r2 < cut2f;
+  return fp64_r2(d, R, d.perm[s], d.perm[t]) < cut2;
+}
+
+template <int ND, bool MULTI>
+__global__ void __launch_bounds__(256) k_density(DevState d) {
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const float4 pi = d.spos[s];
+  const uint32_t c = d.scell[s] - R.cell_off;
+  const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+  const int cx = (int)(c % (uint32_t)nx);
+  const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
+  const int cz = (int)(c / (uint32_t)(nx * ny));
+  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
+  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
+  const uint32_t W = (uint32_t)d.W;
+  uint32_t* lst = d.nlist + (size_t)(s >> 6) * W * 64u + (s & 63u);
+  float rho[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) rho[k] = 0.f;
+  uint32_t cnt = 0;
+  for (int oz = -1; oz <= 1; ++oz) {
+    int qz = cz + oz;
+    qz += (qz < 0) ? nz : 0;
+    qz -= (qz >= nz) ? nz : 0;
+    const float bz = pi.z - (float)oz * R.hf[2];
+    for (int oy = -1; oy <= 1; ++oy) {
+      int qy = cy + oy;
+      qy += (qy < 0) ? ny : 0;
+      qy -= (qy >= ny) ? ny : 0;
+      const float by = pi.y - (float)oy * R.hf[1];
+      for (int ox = -1; ox <= 1; ++ox) {
+        int qx = cx + ox;
+        qx += (qx < 0) ? nx : 0;
+        qx -= (qx >= nx) ? nx : 0;
+        const float bx = pi.x - (float)ox * R.hf[0];
+        const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
+        const uint32_t t0 = d.cell_start[q], t1 = d.cell_start[q + 1];
+        const uint32_t code = (uint32_t)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
+        for (uint32_t t = t0; t < t1; ++t) {
+          if (t == s) continue;
+          const float4 pj = d.spos[t];
+          const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 >= cut_hi) continue;
+          const bool in_rc = decide(r2, 1.0f, 1.0, d, R, s, t);
+          bool in_rd = decide(r2, rd2f, R.rd2, d, R, s, t);
+          if (in_rd && r2 < 1e-6f) in_rd = fp64_r2(d, R, d.perm[s], d.perm[t]) != 0.0;
+          if (in_rd) {
+            const float rr = __builtin_amdgcn_sqrtf(r2);
+            const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
+            if (ND == 1) rho[0] += crho * w * w;
+            else {
+              const uint32_t tj = __float_as_uint(pj.w) & 3u;
+              const float v = crho * w * w;
+#pragma unroll
+              for (int k = 0; k < ND; ++k) rho[k] += (tj == (uint32_t)k) ? v : 0.f;
+            }
+          }
+          if (in_rc) {
+            if (cnt < W) lst[(size_t)cnt * 64u] = t | code;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+  d.ncount[s] = cnt;
+  if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
+}
+
+template <int ND, bool MULTI>
+static void dens_launch(const DevState& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL((k_density<ND, MULTI>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+}
+
+void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_t st) {
+  if (!d.n) return;
+  const bool multi = d.n_rep > 1;
+  if (nd == 1) multi ? dens_launch<1, true>(d, block, st) : dens_launch<1, false>(d, block, st);
+  else multi ? dens_launch<3, true>(d, block, st) : dens_launch<3, false>(d, block, st);
+}
+
+// ---------------------------------------------------------------- force
+// pairs with 1 - r below this use an FP64 1 - r under the sqrt envelope
+constexpr float kNearCut = 2.0e-3f;
+
+template <int NS>
+__device__ __forceinline__ float sel(const float (&v)[NS], uint32_t t) {
+  if (NS == 1) return v[0];
+  return t == 0u ? v[0] : (t == 1u ? v[1] : v[2]);
+}
+
+template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
+__global__ void __launch_bounds__(256) k_force(DevState d) {
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const uint32_t seed3 = R.pair_seed + *d.d_time;
+  const float4 pi = d.spos[s];
+  const float4 vi = d.svel[s];
+  const uint32_t wi = __float_as_uint(pi.w);
+  const uint32_t idi = wi >> 2, ti = NS == 1 ? 0u : (wi & 3u);
+  float rho_i[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) rho_i[k] = d.sdens[(size_t)k * d.cap + s];
+  float A0[NS], C1[NS], SG[NS], GM[NS], MB[ND][NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    A0[q] = R.a0[ti][q];
+    C1[q] = R.c1[ti][q];
+    SG[q] = R.sigma[ti][q];
+    GM[q] = R.gamma[ti][q];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) MB[k][q] = R.mb[ti][q][k];
+  }
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
+  const float arc = R.arc, inv_rdm = R.inv_rdm;
+  const bool rdm1 = R.rdm_is_one != 0;
+  const uint32_t W = (uint32_t)d.W;
+  uint32_t cnt = d.ncount[s];
+  cnt = cnt < W ? cnt : W;
+  const uint32_t* lst = d.nlist + (size_t)(s >> 6) * W * 64u + (s & 63u);
+  float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t e = lst[(size_t)k * 64u];
+    const uint32_t t = e & kIndexMask;
+    const uint32_t code = e >> kCodeShift;
+    const float4 pj = d.spos[t];
+    const float4 vj = d.svel[t];
+    const float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+    const float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
+    const float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rr = r2 * rinv;
+    const uint32_t wj = __float_as_uint(pj.w);
+    const uint32_t idj = wj >> 2, tj = NS == 1 ? 0u : (wj & 3u);
+    Saru sr = saru_seed(idi < idj ? idi : idj, idi < idj ? idj : idi, seed3);
+    const float xi = saru_nrml_f32(sr);
+    float wc = fmaxf(1.0f - rr, 0.0f);
+    if (ENV == 0 && wc < kNearCut) {
+      // sqrt(1 - r) magnifies the FP32 error of r next to the cutoff; take
+      // 1 - r there from the FP64 master positions
+      const double r2d = fp64_r2(d, R, d.perm[s], d.perm[t]);
+      wc = (float)fmax((1.0 - r2d) / (1.0 + sqrt(r2d)), 0.0);
+    }
+    float wR, wD;
+    if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
+    else { wR = wc; wD = wc * wc; }
+    const float cons_c = sel<NS>(A0, tj) + sel<NS>(C1, tj) * fmaxf(rr - arc, 0.0f);
+    float mbs = 0.f;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) mbs += sel<NS>(MB[k], tj) * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
+    const float wm = rdm1 ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
+    const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+    const float fs = (cons_c * wc + mbs * wm + sel<NS>(SG, tj) * wR * xi -
+                      sel<NS>(GM, tj) * wD * rv * rinv) * rinv;
+    fx += fs * dx;
+    fy += fs * dy;
+    fz += fs * dz;
+    if (VIRIAL) {
+      px += fs * dx * dx;
+      py += fs * dy * dy;
+      pz += fs * dz * dz;
+    }
+  }
+  const uint32_t i = d.perm[s];
+  d.acc[i] = fx;
+  d.acc[d.cap + i] = fy;
+  d.acc[2 * d.cap + i] = fz;
+  if (VIRIAL) {
+    d.press[i] = px;
+    d.press[d.cap + i] = py;
+    d.press[2 * d.cap + i] = pz;
+  }
+  if (!isfinite(fx + fy + fz)) atomicOr(d.flags, FLAG_NONFINITE);
+}
+
+template <int NS, int ND, int ENV>
+static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st) {
+  const dim3 g(div_up(d.n, block)), b(block);
+  const bool multi = d.n_rep > 1;
+  if (multi) {
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d);
+  } else {
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d);
+  }
+}
+
+template <int NS, int ND>
+static void force_launch2(const DevState& d, int env, bool virial, int block, hipStream_t st) {
+  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st);
+  else force_launch3<NS, ND, 1>(d, virial, block, st);
+}
+
+void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
+                  hipStream_t st) {
+  if (!d.n) return;
+  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st);
+  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st);
+  else force_launch2<3, 1>(d, env, virial, block, st);
+}
+
+// ---------------------------------------------------------------- pair listing
+// Every ordered pair with r^2 < cutoff^2 as (bead slot i, bead slot j), for
+// the neighbour-set checks; same decision code as the density pass.
+__global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, double cut2,
+                                                    uint32_t* __restrict__ oi,
+                                                    uint32_t* __restrict__ oj,
+                                                    unsigned long long* __restrict__ counter,
+                                                    uint64_t cap) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const float4 pi = d.spos[s];
+  const uint32_t c = d.scell[s] - R.cell_off;
+  const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+  const int cx = (int)(c % (uint32_t)nx), cy = (int)((c / (uint32_t)nx) % (uint32_t)ny),
+            cz = (int)(c / (uint32_t)(nx * ny));
+  for (int oz = -1; oz <= 1; ++oz)
+    for (int oy = -1; oy <= 1; ++oy)
+      for (int ox = -1; ox <= 1; ++ox) {
+        const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
+        const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
+        const float bx = pi.x - (float)ox * R.hf[0], by = pi.y - (float)oy * R.hf[1],
+                    bz = pi.z - (float)oz * R.hf[2];
+        for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) {
+          if (t == s) continue;
+          const float4 pj = d.spos[t];
+          const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 >= cut2f + kGuard) continue;
+          if (decide(r2, cut2f, cut2, d, R, s, t)) {
+            const unsigned long long k = atomicAdd(counter, 1ull);
+            if (k < cap) {
+              oi[k] = d.perm[s];
+              oj[k] = d.perm[t];
+            }
+          }
+        }
+      }
+}
+
+void launch_pairs_list(const DevState& d, double cutoff, uint32_t* out_i, uint32_t* out_j,
+                       unsigned long long* counter, uint64_t cap, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_pairs_list, dim3(div_up(d.n, 256)), dim3(256), 0, st, d,
+                     (float)(cutoff * cutoff), cutoff * cutoff, out_i, out_j, counter, cap);
+}
+
+}  // namespace mdpd

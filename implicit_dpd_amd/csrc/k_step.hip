@@ -1,0 +1,792 @@
+// Integrator, cell-list rebuild, bonded forces, reactions and reductions.
+//
+// FP64 kernels that must agree bit for bit with the oracle's restatement of
+// the reference (drift, kick, bonded, the FP64 pair re-decision) are
+// compiled without FMA contraction (the pragma at the top of each body).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.hpp"
+#include "saru.hpp"
+#include "state.hpp"
+
+namespace mdpd {
+
+// stream tags of the keyed chemistry draws
+constexpr uint32_t TAG_CONE = 0x9E3779B9u;
+constexpr uint32_t TAG_TANG = 0x5bd1e995u;
+constexpr uint32_t TAG_VEL = 0x68e31da4u;
+
+static inline dim3 grid1(uint32_t n, uint32_t b) { return dim3(div_up(n ? n : 1, b)); }
+
+// ---------------------------------------------------------------- fill
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = v;
+}
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st) {
+  if (!n) return;
+  const uint32_t b = 256;
+  uint32_t g = div_up(n, b);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(b), 0, st, p, n, v);
+}
+
+__global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------- cells
+__device__ __forceinline__ uint32_t cell_of(const Replica& R, const double p[3]) {
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int x = (int)floor(p[a] * R.inv_h[a]);
+    x = x < 0 ? 0 : x;
+    x = x >= R.nc[a] ? R.nc[a] - 1 : x;
+    c[a] = x;
+  }
+  return R.cell_off + ((uint32_t)c[2] * (uint32_t)R.nc[1] + (uint32_t)c[1]) * (uint32_t)R.nc[0] +
+         (uint32_t)c[0];
+}
+
+// drift_and_predict, driftkick.hpp:5-30.  vel is left as the last kicked
+// velocity; the predicted velocity (vel_buf == vel of the reference after
+// the drift) goes to vbuf.  With `keys` the cell key and the cell
+// histogram of the rebuild are produced in the same pass.
+__global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  const double dt = R.dt;
+  const uint32_t cap = d.cap;
+  double p[3];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double v = d.vel[a * cap + i];
+    const double ac = (double)d.acc[a * cap + i];
+    if (!predict_only) {
+      const double dr = dt * (v + 0.5 * dt * ac);
+      const double tr = d.pos[a * cap + i] + dr;
+      d.delta[a * cap + i] += dr;
+      p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
+      d.pos[a * cap + i] = p[a];
+    } else {
+      p[a] = d.pos[a * cap + i];
+    }
+    d.vbuf[a * cap + i] = v + 0.5 * dt * ac;
+    fin = fin && isfinite(p[a]) && isfinite(v);
+  }
+  if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
+  if (keys) {
+    const uint32_t k = cell_of(R, p);
+    d.key[i] = k;
+    atomicAdd(&d.cell_cnt[k], 1u);
+  }
+}
+void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, keys ? 1 : 0,
+                     predict_only ? 1 : 0);
+}
+
+__global__ void __launch_bounds__(256) k_keys(DevState d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  double p[3];
+  for (int a = 0; a < 3; ++a) p[a] = d.pos[a * d.cap + i];
+  const uint32_t k = cell_of(R, p);
+  d.key[i] = k;
+  atomicAdd(&d.cell_cnt[k], 1u);
+}
+void launch_keys(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+// exclusive scan of the cell histogram (1024 cells per block, three passes)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// block of 256 threads: exclusive scan of per-thread totals, returns block total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& excl) {
+  __shared__ uint32_t wsum[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (k < w) off += wsum[k];
+    tot += wsum[k];
+  }
+  excl = off + inc - v;
+  __syncthreads();
+  return tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan1(const uint32_t* __restrict__ in,
+                                               uint32_t* __restrict__ out,
+                                               uint32_t* __restrict__ bsum, uint32_t m) {
+  const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+  uint32_t v[4], t = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < m) ? in[base + k] : 0u;
+    t += v[k];
+  }
+  uint32_t excl;
+  const uint32_t tot = block_excl_scan(t, excl);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < m) out[base + k] = excl;
+    excl += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint32_t nb) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += 1024u) {
+    const uint32_t b0 = base + threadIdx.x * 4u;
+    uint32_t v[4], t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = (b0 + k < nb) ? bsum[b0 + k] : 0u;
+      t += v[k];
+    }
+    uint32_t excl;
+    const uint32_t tot = block_excl_scan(t, excl);
+    excl += carry;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (b0 + k < nb) bsum[b0 + k] = excl;
+      excl += v[k];
+    }
+    carry += tot;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
+                                               uint32_t* __restrict__ cursor,
+                                               const uint32_t* __restrict__ bsum, uint32_t m,
+                                               uint32_t total) {
+  const uint32_t base = blockIdx.x * 1024u;
+  const uint32_t add = bsum[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < 1024u; k += 256u) {
+    const uint32_t idx = base + k;
+    if (idx < m) {
+      const uint32_t v = out[idx] + add;
+      out[idx] = v;
+      cursor[idx] = v;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total;
+}
+
+void launch_scan_cells(const DevState& d, hipStream_t st) {
+  const uint32_t m = d.ncell;
+  const uint32_t nb = div_up(m, 1024u);
+  hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, st, d.cell_cnt, d.cell_start, d.bsum, m);
+  hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb);
+  hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, st, d.cell_start, d.cursor, d.bsum, m, d.n);
+}
+
+__global__ void __launch_bounds__(256) k_scatter(DevState d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t slot = atomicAdd(&d.cursor[d.key[i]], 1u);
+  d.perm[slot] = i;
+}
+void launch_scatter(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+// ascending bead slots inside each cell, so the sorted order (and every FP32
+// sum over it) does not depend on the order of the scatter atomics
+__global__ void __launch_bounds__(256) k_cell_sort(DevState d) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.ncell) return;
+  const uint32_t b = d.cell_start[c], e = d.cell_start[c + 1];
+  for (uint32_t k = b + 1; k < e; ++k) {
+    const uint32_t v = d.perm[k];
+    uint32_t q = k;
+    while (q > b) {
+      const uint32_t u = d.perm[q - 1];
+      if (u <= v) break;
+      d.perm[q] = u;
+      --q;
+    }
+    d.perm[q] = v;
+  }
+}
+void launch_cell_sort(const DevState& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d);
+}
+
+// sorted FP32 copies: position in the bead's own cell, predicted velocity
+__global__ void __launch_bounds__(256) k_gather(DevState d, const double* __restrict__ vsrc) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const uint32_t i = d.perm[s];
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const uint32_t key = d.key[i];
+  const uint32_t c = key - R.cell_off;
+  const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
+  const uint32_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+  const uint32_t cap = d.cap;
+  const float lx = (float)(d.pos[i] - (double)cx * R.h[0]);
+  const float ly = (float)(d.pos[cap + i] - (double)cy * R.h[1]);
+  const float lz = (float)(d.pos[2 * cap + i] - (double)cz * R.h[2]);
+  const uint32_t w = (d.id[i] << 2) | (d.prop[i] & 3u);
+  d.spos[s] = make_float4(lx, ly, lz, __uint_as_float(w));
+  d.svel[s] = make_float4((float)vsrc[i], (float)vsrc[cap + i], (float)vsrc[2 * cap + i], 0.f);
+  d.scell[s] = key;
+}
+void launch_gather(const DevState& d, const double* vsrc, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_gather, grid1(d.n, 256), dim3(256), 0, st, d, vsrc);
+}
+
+// ---------------------------------------------------------------- kick
+// driftkick.hpp:32-40: vel = vel_buf + dt * 0.5 * acc
+__global__ void __launch_bounds__(256) k_kick(DevState d) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const double dt = d.reps[r].dt;
+  const uint32_t cap = d.cap;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    d.vel[a * cap + i] = d.vbuf[a * cap + i] + dt * 0.5 * (double)d.acc[a * cap + i];
+}
+void launch_kick(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_kick, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+// ---------------------------------------------------------------- reductions
+// Neumaier-compensated FP64 sums; fixed grid and fixed trees, so the result
+// is the same on every run.
+struct KSum {
+  double s, c;
+  __device__ __forceinline__ void add(double x) {
+#pragma clang fp contract(off)
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void merge(const KSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+};
+
+constexpr int kRedBlocks = 64;   // partial blocks per replica
+constexpr int kRedThreads = 256;
+
+template <int Q>
+__device__ __forceinline__ void block_reduce_write(KSum (&acc)[Q], double* __restrict__ out2) {
+  // out2[q*2 + {0,1}] = (s, c) of the block
+  __shared__ double sh_s[Q][kRedThreads / 64];
+  __shared__ double sh_c[Q][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    KSum v = acc[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      KSum u;
+      u.s = __shfl_down(v.s, o, 64);
+      u.c = __shfl_down(v.c, o, 64);
+      if (lane < o) v.merge(u);
+    }
+    if (lane == 0) { sh_s[q][w] = v.s; sh_c[q][w] = v.c; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      KSum v{sh_s[q][0], sh_c[q][0]};
+      for (int k = 1; k < kRedThreads / 64; ++k) v.merge(KSum{sh_s[q][k], sh_c[q][k]});
+      out2[2 * q] = v.s;
+      out2[2 * q + 1] = v.c;
+    }
+  }
+}
+
+// final: one block per replica sums its kRedBlocks partials in order
+template <int Q>
+__global__ void k_red_final(const double* __restrict__ part, double* __restrict__ out) {
+  const int r = blockIdx.x;
+  if (threadIdx.x < Q) {
+    const int q = threadIdx.x;
+    KSum v{0.0, 0.0};
+    for (int b = 0; b < kRedBlocks; ++b) {
+      const double* p = part + ((size_t)r * kRedBlocks + b) * 2 * Q;
+      v.merge(KSum{p[2 * q], p[2 * q + 1]});
+    }
+    out[(size_t)r * Q + q] = v.s + v.c;
+  }
+}
+
+// sum of velocities per replica (for remove_cmdrift_global)
+__global__ void __launch_bounds__(kRedThreads) k_cm_sum(DevState d) {
+  const int r = blockIdx.y;
+  const uint32_t b0 = d.rep_off[r], n_r = d.rep_off[r + 1] - b0;
+  KSum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < n_r; k += kRedBlocks * kRedThreads) {
+    const uint32_t i = b0 + k;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a].add(d.vel[a * d.cap + i]);
+  }
+  block_reduce_write<3>(acc, d.red + ((size_t)r * kRedBlocks + blockIdx.x) * 6);
+}
+
+__global__ void __launch_bounds__(256) k_cm_apply(DevState d) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const double n_r = (double)(d.rep_off[r + 1] - d.rep_off[r]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d.vel[a * d.cap + i] -= d.red_out[(size_t)r * 3 + a] / n_r;
+}
+
+void launch_cm_drift(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_cm_sum, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d);
+  hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, d.red_out);
+  hipLaunchKernelGGL(k_cm_apply, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+// Observer sums per replica (observer.hpp:82-178); amp_num[r] is the
+// replica's amphiphile count.  Quantities:
+//  0-2 sum v_a^2; 3-5 sum (0.5 press_a + v_a^2); 6-8 sum press_a;
+//  9 sum |delta|^2; 10 sum |delta|^2 over local index >= amp_num (the
+//  reference's solvent loop, observer.hpp:171); 11 sum over amphiphiles of
+//  |mean delta of beads 4a..4a+3|^2 (observer.hpp:160-167).
+__global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
+                                                         const uint32_t* __restrict__ amp_num,
+                                                         double* __restrict__ part) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.y;
+  const uint32_t b0 = d.rep_off[r], n_r = d.rep_off[r + 1] - b0;
+  const uint32_t na = amp_num[r];
+  const uint32_t cap = d.cap;
+  KSum acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) acc[q] = KSum{0.0, 0.0};
+  for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < n_r; k += kRedBlocks * kRedThreads) {
+    const uint32_t i = b0 + k;
+    double dd = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = d.vel[a * cap + i];
+      const double pr = (double)d.press[a * cap + i];
+      const double dl = d.delta[a * cap + i];
+      acc[a].add(v * v);
+      acc[3 + a].add(pr * 0.5 + v * v);
+      acc[6 + a].add(pr);
+      dd += dl * dl;
+    }
+    acc[9].add(dd);
+    if (k >= na) acc[10].add(dd);
+    if ((k & 3u) == 0 && (k >> 2) < na && k + 3 < n_r) {
+      double m[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        m[a] = d.delta[a * cap + i] + d.delta[a * cap + i + 1];
+        m[a] += d.delta[a * cap + i + 2];
+        m[a] += d.delta[a * cap + i + 3];
+        m[a] /= 4.0;
+      }
+      acc[11].add(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    }
+  }
+  block_reduce_write<12>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 2 * 12);
+}
+
+void launch_observe(const DevState& d, const uint32_t* amp_num, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_observe, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d, amp_num,
+                     d.red);
+  hipLaunchKernelGGL(k_red_final<12>, dim3(d.n_rep), dim3(64), 0, st, d.red, out);
+}
+
+// ---------------------------------------------------------------- bonded
+// ForceBonded::CalcBondBendGlobalCell<4>, bdf_calculator.hpp:153-189, with
+// MinImage (:55-59), StoreBondForceWithARLaw (:95-115) and
+// StoreBendForceWithARLaw (:117-150), FP64, one thread per chain.  Chains
+// are disjoint, so the read-modify-write of acc needs no atomics.
+__device__ __forceinline__ void bd_min_image(const Replica& R, double d[3]) {
+#pragma clang fp contract(off)
+  for (int a = 0; a < 3; ++a) d[a] -= R.box[a] * round(d[a] * R.inv_box[a]);
+}
+
+
+__global__ void __launch_bounds__(256) k_bonded(DevState d, int virial, double* __restrict__ vir_chain) {
+#pragma clang fp contract(off)
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.n_amp) return;
+  const Replica& R = d.reps[d.amp_rep[a]];
+  const uint32_t cap = d.cap;
+  uint32_t l[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) l[u] = d.topo[4 * a + u];
+  const double cf_b = d.cf_bend[a];
+  double F[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[u][c] = 0.0;
+  double dr[3][3], inv_dr[3], vir[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 1; u < 4; ++u) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dr[u - 1][c] = d.pos[c * cap + l[u]] - d.pos[c * cap + l[u - 1]];
+    bd_min_image(R, dr[u - 1]);
+    const double dist2 = dr[u - 1][0] * dr[u - 1][0] + dr[u - 1][1] * dr[u - 1][1] +
+                         dr[u - 1][2] * dr[u - 1][2];
+    inv_dr[u - 1] = 1.0 / sqrt(dist2);
+    // bond, bdf_calculator.hpp:95-115
+    const double cf_bond = R.cf_s * (inv_dr[u - 1] - R.ibond);
+    double Fb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Fb[c] = cf_bond * dr[u - 1][c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vir[c] += 2.0 * dr[u - 1][c] * Fb[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      F[u - 1][c] -= Fb[c];
+      F[u][c] += Fb[c];
+    }
+    if (u >= 2) {
+      // bend over beads u-2, u-1, u, bdf_calculator.hpp:117-150
+      const double* d0 = dr[u - 2];
+      const double* d1 = dr[u - 1];
+      const double inv_dr_prod = inv_dr[u - 2] * inv_dr[u - 1];
+      const double inv_dist0 = inv_dr[u - 2] * inv_dr[u - 2];
+      const double inv_dist1 = inv_dr[u - 1] * inv_dr[u - 1];
+      const double in_prod = d0[0] * d1[0] + d0[1] * d1[1] + d0[2] * d1[2];
+      const double cf_bd = cf_b * inv_dr_prod;
+      const double crs0 = in_prod * inv_dist0, crs1 = in_prod * inv_dist1;
+      double T0[3], T1[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T0[c] = cf_bd * (d1[c] - crs0 * d0[c]);
+        T1[c] = cf_bd * (d0[c] - crs1 * d1[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vir[c] += 2.0 * (d0[c] * T0[c] + d1[c] * T1[c]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        F[u - 2][c] -= T0[c];
+        F[u - 1][c] += T0[c] - T1[c];
+        F[u][c] += T1[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float* p = &d.acc[c * cap + l[u]];
+      *p = (float)((double)*p + F[u][c]);
+    }
+  if (virial) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vir_chain[3 * (size_t)a + c] = vir[c];
+  }
+}
+
+// per-replica sum of the chain virials (chains of a replica are contiguous)
+__global__ void __launch_bounds__(kRedThreads) k_bonded_vir(const double* __restrict__ vir_chain,
+                                                            const uint32_t* __restrict__ amp_off,
+                                                            double* __restrict__ part) {
+  const int r = blockIdx.y;
+  const uint32_t a0 = amp_off[r], na = amp_off[r + 1] - a0;
+  KSum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < na; k += kRedBlocks * kRedThreads)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c].add(vir_chain[3 * (size_t)(a0 + k) + c]);
+  block_reduce_write<3>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 6);
+}
+
+void launch_bonded(const DevState& d, bool virial, double* vir_out, hipStream_t st) {
+  // vir_out: device [n_rep][3]; the chain scratch lives after the partials
+  if (d.n_amp) {
+    double* vir_chain = d.red + (size_t)d.n_rep * kRedBlocks * 2 * 16;
+    hipLaunchKernelGGL(k_bonded, grid1(d.n_amp, 256), dim3(256), 0, st, d, virial ? 1 : 0,
+                       vir_chain);
+    if (virial) {
+      hipLaunchKernelGGL(k_bonded_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st,
+                         vir_chain, d.amp_off, d.red);
+      hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out);
+    }
+  } else if (virial) {
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------- reactions
+// head conversion: unit-0 bead of type conv_from -> conv_to when
+// Saru(amp_id, time, chem_seed).uni_open() <= conv_p
+__global__ void __launch_bounds__(256) k_head_conv(DevState d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  if (d.unit[i] != 0u) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  if (R.conv_p <= 0.0 || d.prop[i] != (uint32_t)R.conv_from) return;
+  Saru s = saru_seed(d.amp_id[i], *d.d_time, R.chem_seed);
+  if (saru_uni_f64(s) <= R.conv_p) d.prop[i] = (uint32_t)R.conv_to;
+}
+void launch_head_conversion(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_head_conv, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+__global__ void k_time_inc(uint32_t* t) { *t += 1u; }
+void launch_time_inc(uint32_t* d_time, hipStream_t st) {
+  hipLaunchKernelGGL(k_time_inc, dim3(1), dim3(1), 0, st, d_time);
+}
+
+// bead counts along one axis per replica
+__global__ void __launch_bounds__(256) k_profile(DevState d, int axis, int nbins, uint32_t* hist) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  int b = (int)floor(d.pos[axis * d.cap + i] * R.inv_box[axis] * nbins);
+  b = b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
+  atomicAdd(&hist[(size_t)r * nbins + b], 1u);
+}
+void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_profile, grid1(d.n, 256), dim3(256), 0, st, d, axis, nbins, hist);
+}
+
+// ---------------------------------------------------------------- neighbour CM
+// CalcDensity's centre-of-mass sums (nbdf_calculator.hpp:22-58) for a list
+// of beads, FP64 from the master positions: hydrophilic CM of the bead and
+// its Hyphil neighbours with r^2 < 1, hydrophobic CM of the Hyphob
+// neighbours with r^2 < 1.44, j at its image position r_i - d.
+__device__ __forceinline__ double fp64_disp(const DevState& d, const Replica& R, uint32_t i,
+                                            uint32_t j, double dv[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double x = d.pos[a * d.cap + i] - d.pos[a * d.cap + j];
+    if (x > 0.5 * R.box[a]) x -= R.box[a];
+    else if (x < -0.5 * R.box[a]) x += R.box[a];
+    dv[a] = x;
+  }
+  return dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+}
+
+__global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, uint32_t nb,
+                              double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const uint32_t i = beads[k];
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  double ri[3], hl[3], hb[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 3; ++a) { ri[a] = d.pos[a * d.cap + i]; hl[a] = ri[a]; }
+  uint32_t cl = 1, cb = 0;
+  int rad[3];
+  bool brute = false;
+  for (int a = 0; a < 3; ++a) {
+    rad[a] = (int)ceil(1.2 * R.inv_h[a] - 1e-12);
+    if (2 * rad[a] + 1 > R.nc[a]) brute = true;
+  }
+  auto visit = [&](uint32_t j) {
+    if (j == i) return;
+    double dv[3];
+    const double r2 = fp64_disp(d, R, i, j, dv);
+    if (r2 == 0.0) return;
+    const uint32_t pj = d.prop[j];
+    if (r2 < 1.44 && pj == 1u) { for (int a = 0; a < 3; ++a) hb[a] += ri[a] - dv[a]; cb++; }
+    if (r2 < 1.0 && pj == 0u) { for (int a = 0; a < 3; ++a) hl[a] += ri[a] - dv[a]; cl++; }
+  };
+  if (brute) {
+    const uint32_t b0 = d.rep_off[r], b1 = d.rep_off[r + 1];
+    for (uint32_t j = b0; j < b1; ++j) visit(j);
+  } else {
+    const uint32_t c = d.key[i] - R.cell_off;
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    const int cx = (int)(c % (uint32_t)nx), cy = (int)((c / (uint32_t)nx) % (uint32_t)ny),
+              cz = (int)(c / (uint32_t)(nx * ny));
+    for (int oz = -rad[2]; oz <= rad[2]; ++oz)
+      for (int oy = -rad[1]; oy <= rad[1]; ++oy)
+        for (int ox = -rad[0]; ox <= rad[0]; ++ox) {
+          const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
+          const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
+          for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) visit(d.perm[t]);
+        }
+  }
+  for (int a = 0; a < 3; ++a) {
+    out[8 * (size_t)k + a] = hl[a] / (double)cl;
+    out[8 * (size_t)k + 3 + a] = hb[a] / (double)cb;
+  }
+  out[8 * (size_t)k + 6] = (double)cl;
+  out[8 * (size_t)k + 7] = (double)cb;
+}
+void launch_neighbor_cm(const DevState& d, const uint32_t* beads, uint32_t nb, double* out,
+                        hipStream_t st) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_neighbor_cm, grid1(nb, 64), dim3(64), 0, st, d, beads, nb, out);
+}
+
+// ---------------------------------------------------------------- Saru check
+__global__ void k_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, uint32_t n,
+                            uint32_t* u32out, float* nrml_out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  Saru a = saru_seed(s1[k], s2[k], s3[k]);
+  for (int q = 0; q < 8; ++q) u32out[8 * (size_t)k + q] = saru_u32(a);
+  Saru b = saru_seed(s1[k], s2[k], s3[k]);
+  nrml_out[k] = saru_nrml_f32(b);
+}
+void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, uint32_t n,
+                      uint32_t* u32out, float* nrml_out, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_saru_test, grid1(n, 256), dim3(256), 0, st, s1, s2, s3, n, u32out, nrml_out);
+}
+
+// ---------------------------------------------------------------- chemistry
+__global__ void k_find_ids(DevState d, const uint32_t* ids, uint32_t nid, uint32_t* idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t v = d.id[i];
+  for (uint32_t k = 0; k < nid; ++k)
+    if (v == ids[k]) idx[k] = i;
+}
+void launch_find_ids(const DevState& d, const uint32_t* ids, uint32_t nid, uint32_t* idx,
+                     hipStream_t st) {
+  if (!d.n || !nid) return;
+  hipLaunchKernelGGL(k_find_ids, grid1(d.n, 256), dim3(256), 0, st, d, ids, nid, idx);
+}
+
+// DetermineTargetIdWithCorePos loop body (chemmanager.hpp:140-161) for every
+// amphiphile j and core c; core[c] = (corrected head position, h2t).
+__global__ void k_chem_flags(DevState d, const double* __restrict__ core, uint32_t n_core,
+                             double p_thresld, double influ_rad, double influ_hei,
+                             double influ_dep, double influ_grd, uint32_t time,
+                             uint8_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = blockIdx.y;
+  if (j >= d.n_amp || c >= n_core) return;
+  const Replica& R = d.reps[0];
+  const double* ch = core + 6 * (size_t)c;
+  const double* h2t = ch + 3;
+  const uint32_t head = d.topo[4 * j];
+  double c2p[3];
+  for (int a = 0; a < 3; ++a) c2p[a] = d.pos[a * d.cap + head] - ch[a];
+  bd_min_image(R, c2p);
+  const double depth = h2t[0] * c2p[0] + h2t[1] * c2p[1] + h2t[2] * c2p[2];
+  double tg[3];
+  for (int a = 0; a < 3; ++a) tg[a] = c2p[a] - depth * h2t[a];
+  const double rad = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+  const double rad_thr = influ_grd * depth + influ_rad;
+  double rnd = 2.0;
+  if ((rad <= rad_thr) && (depth >= -influ_hei) && (depth <= influ_dep)) {
+    Saru s = saru_seed(j, time, R.chem_seed + TAG_CONE * (c + 1u));
+    rnd = saru_uni_f64(s);
+  }
+  flags[(size_t)c * d.n_amp + j] = (rnd <= p_thresld) ? 1 : 0;
+}
+void launch_chem_flags(const DevState& d, const double* core, uint32_t n_core, double p_thresld,
+                       double influ_rad, double influ_hei, double influ_dep, double influ_grd,
+                       uint32_t time, uint8_t* flags, hipStream_t st) {
+  if (!d.n_amp || !n_core) return;
+  hipLaunchKernelGGL(k_chem_flags, dim3(div_up(d.n_amp, 256), n_core), dim3(256), 0, st, d, core,
+                     n_core, p_thresld, influ_rad, influ_hei, influ_dep, influ_grd, time, flags);
+}
+
+// RandomChemEvent insertion loop (chemmanager.hpp:258-271) with CreateAmpPart
+// (:35-67): target t = (head slot, tail slot) makes amphiphile n_amp + t of
+// beads n + 4t + u.  One thread per new bead.
+__global__ void k_chem_insert(DevState d, const uint32_t* __restrict__ targets, uint32_t ntarget,
+                              double eps, double bond_leng, uint32_t time) {
+#pragma clang fp contract(off)
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 4 * ntarget) return;
+  const uint32_t t = g >> 2, u = g & 3u;
+  const Replica& R = d.reps[0];
+  const uint32_t hd = targets[2 * t], tl = targets[2 * t + 1];
+  const uint32_t cap = d.cap;
+  double h2e[3];
+  for (int a = 0; a < 3; ++a) h2e[a] = d.pos[a * cap + tl] - d.pos[a * cap + hd];
+  bd_min_image(R, h2e);
+  double l = sqrt(h2e[0] * h2e[0] + h2e[1] * h2e[1] + h2e[2] * h2e[2]);
+  for (int a = 0; a < 3; ++a) h2e[a] /= l;
+  const uint32_t new_amp = d.n_amp + t;
+  Saru st = saru_seed(new_amp, time, R.chem_seed ^ TAG_TANG);
+  const double theta = 2.0 * M_PI * saru_uni_f64(st);
+  double base[3] = {cos(theta), sin(theta), 0.0};
+  const double bd = base[0] * h2e[0] + base[1] * h2e[1] + base[2] * h2e[2];
+  double tv[3];
+  for (int a = 0; a < 3; ++a) tv[a] = base[a] - h2e[a] * bd;
+  l = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+  for (int a = 0; a < 3; ++a) tv[a] /= l;
+  const uint32_t i = d.n + g;
+  for (int a = 0; a < 3; ++a) {
+    tv[a] *= eps;
+    h2e[a] *= bond_leng;
+    double x = (tv[a] + d.pos[a * cap + hd]) + h2e[a] * (double)u;
+    x -= floor(x * R.inv_box[a]) * R.box[a];
+    d.pos[a * cap + i] = x;
+  }
+  Saru sv = saru_seed(i, time, R.chem_seed ^ TAG_VEL);
+  for (int a = 0; a < 3; ++a) {
+    const double u1 = saru_uni_f64(sv);
+    const double u2 = saru_uni_f64(sv);
+    d.vel[a * cap + i] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    d.vbuf[a * cap + i] = 0.0;
+    d.acc[a * cap + i] = 0.0f;
+    d.press[a * cap + i] = 0.0f;
+    d.delta[a * cap + i] = 0.0;
+  }
+  d.id[i] = i;
+  d.prop[i] = (u == 0) ? 0u : 1u;
+  d.amp_id[i] = new_amp;
+  d.unit[i] = u;
+  d.topo[4 * (size_t)new_amp + u] = i;
+  if (u == 0) d.amp_rep[new_amp] = 0;
+}
+void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget, double eps,
+                        double bond_leng, uint32_t time, hipStream_t st) {
+  if (!ntarget) return;
+  hipLaunchKernelGGL(k_chem_insert, grid1(4 * ntarget, 64), dim3(64), 0, st, d, targets, ntarget,
+                     eps, bond_leng, time);
+}
+
+__global__ void k_copy_d(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_copy_d, dim3(1024), dim3(256), 0, st, dst, src, n);
+}
+
+}  // namespace mdpd

@@ -1,0 +1,106 @@
+// Saru counter-based PRNG for the device and the host.
+//
+// Same integer algorithm as the reference's CPU generator
+// (include/saruprng.hpp:208-255 three-seed constructor, :199-202 and
+// :322-334 one-step u32, :427-435 uni_open, :442-445 nrml with the first
+// draw feeding log and the second cos), written here for HIP.  The u32
+// stream is bit-exact; nrml_f32 is an FP32 evaluation whose error against
+// the FP64 nrml is bounded in the tests.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdpd {
+
+struct Saru {
+  uint32_t state, wstate;
+};
+
+__host__ __device__ __forceinline__ uint32_t asr(uint32_t x, int s) {
+  return (uint32_t)(((int32_t)x) >> s);
+}
+
+__host__ __device__ __forceinline__ Saru saru_seed(uint32_t seed1, uint32_t seed2,
+                                                   uint32_t seed3) {
+  seed3 ^= (seed1 << 7) ^ (seed2 >> 6);
+  seed2 += (seed1 >> 4) ^ (seed3 >> 15);
+  seed1 ^= (seed2 << 9) + (seed3 << 8);
+  seed3 ^= 0xA5366B4Du * ((seed2 >> 11) ^ (seed1 << 1));
+  seed2 += 0x72BE1579u * ((seed1 << 4) ^ (seed3 >> 16));
+  seed1 ^= 0x3F38A6EDu * ((seed3 >> 5) ^ asr(seed2, 22));
+  seed2 += seed1 * seed3;
+  seed1 += seed3 ^ (seed2 >> 2);
+  seed2 ^= asr(seed2, 17);
+  Saru s;
+  s.state = 0x79dedea3u * (seed1 ^ asr(seed1, 14));
+  s.wstate = (s.state + seed2) ^ asr(s.state, 8);
+  s.state = s.state + (s.wstate * (s.wstate ^ 0xdddf97f5u));
+  s.wstate = 0xABCB96F7u + (s.wstate >> 1);
+  return s;
+}
+
+__host__ __device__ __forceinline__ uint32_t saru_u32(Saru& s) {
+  s.state = 0x4beb5d59u * s.state + 0x2600e1f7u;
+  s.wstate = s.wstate + 0x8009d14bu + (asr(s.wstate, 31) & 0xda879addu);
+  const uint32_t v = (s.state ^ (s.state >> 26)) + s.wstate;
+  return (v ^ (v >> 20)) * 0x6957f5a7u;
+}
+
+// (0,1] uniform in FP64, exact: ((int)(u >> 1) + 0.5) / 2^31
+__host__ __device__ __forceinline__ double saru_uni_f64(Saru& s) {
+  return ((double)(int32_t)(saru_u32(s) >> 1) + 0.5) * (1.0 / 2147483648.0);
+}
+
+// -ln(u) for u = (k + 0.5) / 2^31, k = x >> 1, in FP32.  Near u = 1 the FP32
+// rounding of u would dominate, so there t = 1 - u is formed exactly from
+// the integer and -ln(1 - t) = t (1 + t (1/2 + t/3)) (|t| <= 2^-8, relative
+// truncation error below 2^-26).
+__device__ __forceinline__ float neg_log_uni(uint32_t x) {
+  const uint32_t k = x >> 1;
+  const float u = ((float)(int32_t)k + 0.5f) * 4.656612873077392578125e-10f;
+  const float g = -0.69314718055994530942f * __builtin_amdgcn_logf(u);
+  const uint32_t m = 0x80000000u - k;  // 1 <= m <= 2^31
+  const float t = ((float)m - 0.5f) * 4.656612873077392578125e-10f;
+  const float near1 = t * (1.0f + t * (0.5f + t * 0.333333333f));
+  return (m <= (1u << 23)) ? near1 : g;
+}
+
+// cos(2 pi v) for v = (k + 0.5) / 2^31 in (0, 1].  The argument is folded to
+// w in [0, 1/4] by symmetry (exactly, in FP32), then cos(2 pi w) or
+// sin(2 pi (1/4 - w)) by Taylor polynomials in z = (2 pi w)^2 / (pi/2 -
+// 2 pi w)^2 evaluated on the shorter side, |error| < 2e-7.
+__device__ __forceinline__ float cos_2pi_uni(uint32_t x) {
+  const uint32_t k = x >> 1;
+  // v = (k + 0.5) / 2^31; fold with integer arithmetic on 2k+1 over 2^32
+  const uint32_t q = 2u * k + 1u;            // v = q / 2^32, q odd
+  // w = distance of v to the nearest multiple of 1/2, sign of cos from
+  // which half-period: cos(2 pi v) = s * cos(2 pi w), w in [0, 1/4]
+  const uint32_t qh = q & 0x7fffffffu;        // v mod 1/2
+  const bool upper = qh > 0x40000000u;        // w = 1/2 - (v mod 1/2)
+  const uint32_t wq = upper ? (0x80000000u - qh) : qh;
+  // sign: cos negative for v in (1/4, 3/4)
+  const bool neg = (q > 0x40000000u) && (q < 0xC0000000u);
+  const float w = (float)wq * 2.3283064365386962890625e-10f;  // in [0, 1/4]
+  const float TWO_PI = 6.28318530717958647692f;
+  float r;
+  if (w <= 0.125f) {
+    const float a = TWO_PI * w;   // [0, pi/4]
+    const float z = a * a;
+    r = 1.0f + z * (-0.5f + z * (4.16666679e-2f + z * (-1.38888892e-3f + z * 2.48015876e-5f)));
+  } else {
+    const float a = TWO_PI * (0.25f - w);  // [0, pi/4]
+    const float z = a * a;
+    r = a * (1.0f + z * (-0.166666672f + z * (8.33333377e-3f + z * (-1.98412701e-4f + z * 2.75573188e-6f))));
+  }
+  return neg ? -r : r;
+}
+
+// FP32 Box-Muller normal of the pair noise: sqrt(-2 ln u1) cos(2 pi u2)
+__device__ __forceinline__ float saru_nrml_f32(Saru& s) {
+  const uint32_t x1 = saru_u32(s);
+  const uint32_t x2 = saru_u32(s);
+  const float r = __builtin_amdgcn_sqrtf(2.0f * neg_log_uni(x1));
+  return r * cos_2pi_uni(x2);
+}
+
+}  // namespace mdpd

@@ -1,0 +1,97 @@
+// Device-state view handed (by value) to the launch wrappers.
+#pragma once
+#include "common.hpp"
+
+namespace mdpd {
+
+struct DevState {
+  // master bead storage, slot order (upload order), SoA with stride cap
+  double* pos;      // [3][cap]
+  double* vel;      // [3][cap]  velocity after the last kick
+  double* vbuf;     // [3][cap]  predicted velocity = vel_buf (driftkick.hpp:25-27)
+  double* delta;    // [3][cap]  delta_sumr
+  float* acc;       // [3][cap]
+  float* press;     // [3][cap]  per-bead pair virial (written on virial steps)
+  uint32_t* id;
+  uint32_t* prop;
+  uint32_t* amp_id;
+  uint32_t* unit;
+  uint32_t cap;
+  uint32_t n;       // beads in use
+  // cell list, rebuilt every step
+  uint32_t* key;        // [cap] global cell of each bead
+  uint32_t* cell_cnt;   // [ncell + 1]
+  uint32_t* cell_start; // [ncell + 1]
+  uint32_t* cursor;     // [ncell]
+  uint32_t* bsum;       // scan block sums
+  uint32_t* perm;       // [cap] sorted slot -> bead slot
+  uint32_t ncell;
+  // sorted FP32 copies
+  float4* spos;         // local coordinates in the bead's cell, .w = id<<2 | type
+  float4* svel;         // predicted velocity
+  uint32_t* scell;      // global cell of each sorted slot
+  float* sdens;         // [nd][cap] density
+  uint32_t* nlist;      // wave-tiled pair list [cap/64][W][64]
+  uint32_t* ncount;     // [cap]
+  int32_t W;
+  // replicas
+  Replica* reps;
+  uint32_t* rep_off;    // [n_rep + 1] first bead of each replica
+  uint32_t* rep_cell_off; // [n_rep + 1]
+  int32_t n_rep;
+  int32_t n_dens;
+  // bonded
+  uint32_t* topo;       // [4][amp_cap] chain bead slots, amp-major
+  double* cf_bend;      // [amp_cap]
+  uint32_t* amp_rep;    // [amp_cap] replica of each chain
+  uint32_t* amp_off;    // [n_rep + 1] first chain of each replica
+  uint32_t n_amp;
+  // scalars and scratch
+  uint32_t* d_time;
+  uint32_t* flags;
+  double* red;          // reduction partials
+  double* red_out;      // reduction results
+};
+
+// launch wrappers (defined in the .hip files)
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st);
+void launch_keys(const DevState& d, hipStream_t st);
+void launch_scan_cells(const DevState& d, hipStream_t st);
+void launch_scatter(const DevState& d, hipStream_t st);
+void launch_cell_sort(const DevState& d, hipStream_t st);
+void launch_gather(const DevState& d, const double* vsrc, hipStream_t st);
+void launch_density(const DevState& d, int ns, int nd, int block, hipStream_t st);
+void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
+                  hipStream_t st);
+void launch_bonded(const DevState& d, bool virial, double* vir_out /*[n_rep][3] device*/,
+                   hipStream_t st);
+void launch_kick(const DevState& d, hipStream_t st);
+void launch_cm_drift(const DevState& d, hipStream_t st);
+void launch_head_conversion(const DevState& d, hipStream_t st);
+void launch_time_inc(uint32_t* d_time, hipStream_t st);
+// observables: out[n_rep][kObsQ] FP64
+constexpr int kObsQ = 16;
+void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, double* out,
+                    hipStream_t st);
+void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist,
+                            hipStream_t st);
+void launch_neighbor_cm(const DevState& d, const uint32_t* beads, uint32_t nb, double* cm_out,
+                        hipStream_t st);
+void launch_pairs_list(const DevState& d, double cutoff, uint32_t* out_i, uint32_t* out_j,
+                       unsigned long long* counter, uint64_t cap, hipStream_t st);
+void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, uint32_t n,
+                      uint32_t* u32out, float* nrml_out, hipStream_t st);
+// chemistry
+void launch_find_ids(const DevState& d, const uint32_t* ids, uint32_t nid, uint32_t* idx_out,
+                     hipStream_t st);
+void launch_chem_flags(const DevState& d, const double* core /*[n_core][6]: pos h, h2t*/,
+                       uint32_t n_core, double p_thresld, double influ_rad, double influ_hei,
+                       double influ_dep, double influ_grd, uint32_t time, uint8_t* flags,
+                       hipStream_t st);
+void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget,
+                        double eps, double bond_leng, uint32_t time, hipStream_t st);
+// snapshot of master state (autotuner)
+void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st);
+
+}  // namespace mdpd

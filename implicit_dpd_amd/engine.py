@@ -1,0 +1,251 @@
+"""Python handle on the engine's C ABI (libmdpd.so), for tests, bench and tools.
+
+The engine itself is the HIP library; this module only marshals numpy arrays
+through ctypes.  Method names follow the reference's driver calls
+(drift_and_predict, CalcDensity, CalcForceEpEpDPD, ForceBonded,
+kick, remove_cmdrift_global, Observer; src/main.cpp:167-208).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import MdpdChemParams, MdpdObservables, MdpdParams, MdpdTuning, check
+
+_DP = C.POINTER(C.c_double)
+_U32P = C.POINTER(C.c_uint32)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_DP)
+
+
+def _u(a):
+    return None if a is None else a.ctypes.data_as(_U32P)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def observables_dict(o):
+    return {
+        "kin_temp": np.array(o.kin_temp[:]),
+        "pressure": np.array(o.pressure[:]),
+        "diffusion": np.array(o.diffusion[:]),
+        "bonded_vir": np.array(o.bonded_vir[:]),
+        "pair_vir": np.array(o.pair_vir[:]),
+        "n_beads": o.n_beads,
+        "n_amp": o.n_amp,
+        "time": o.time,
+    }
+
+
+class Engine:
+    """One engine context: a single system, or an ensemble of replicas."""
+
+    def __init__(self, params, capacity=None, n_beads=None):
+        L = capi.lib()
+        self.L = L
+        h = C.c_void_p()
+        if n_beads is None:
+            self.params = [params]
+            check(L.mdpd_create(C.byref(params), int(capacity), C.byref(h)))
+            self.n_rep = 1
+        else:
+            plist = params if isinstance(params, (list, tuple)) else [params] * len(n_beads)
+            arr = (MdpdParams * len(plist))(*plist)
+            nb = np.ascontiguousarray(n_beads, dtype=np.uint32)
+            check(L.mdpd_create_ensemble(arr, _u(nb), len(plist), C.byref(h)))
+            self.params = list(plist)
+            self.n_rep = len(plist)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.mdpd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return check(rc, self.h)
+
+    # ---- state
+    def upload(self, pos, vel, ids, prop, amp_id=None, unit=None, acc=None):
+        pos, vel = _f64(pos), _f64(vel)
+        n = len(pos)
+        self._c(self.L.mdpd_upload_soa(self.h, n, _d(pos), _d(vel),
+                                       _d(_f64(acc)) if acc is not None else None,
+                                       _u(_u32(ids)), _u(_u32(prop)),
+                                       _u(_u32(amp_id)) if amp_id is not None else None,
+                                       _u(_u32(unit)) if unit is not None else None))
+
+    def upload_fpdpd(self, records):
+        buf = np.ascontiguousarray(records)
+        self._c(self.L.mdpd_upload_fpdpd(self.h, buf.ctypes.data_as(C.c_void_p), len(buf),
+                                         buf.dtype.itemsize))
+
+    def download_fpdpd(self, n=None):
+        from .fpdpd import FPDPD_DTYPE
+        n = self.num_beads() if n is None else n
+        out = np.zeros(n, dtype=FPDPD_DTYPE)
+        self._c(self.L.mdpd_download_fpdpd(self.h, out.ctypes.data_as(C.c_void_p), n,
+                                           FPDPD_DTYPE.itemsize))
+        return out
+
+    def num_beads(self):
+        n = C.c_uint32()
+        self._c(self.L.mdpd_num_beads(self.h, C.byref(n)))
+        return n.value
+
+    def download(self):
+        n = self.num_beads()
+        out = {k: np.zeros((n, 3)) for k in ("pos", "vel", "vel_buf", "acc", "press", "dens",
+                                              "delta")}
+        ui = {k: np.zeros(n, np.uint32) for k in ("id", "prop", "amp_id", "unit")}
+        self._c(self.L.mdpd_download_soa(self.h, n, *[_d(out[k]) for k in (
+            "pos", "vel", "vel_buf", "acc", "press", "dens", "delta")],
+            *[_u(ui[k]) for k in ("id", "prop", "amp_id", "unit")]))
+        out.update(ui)
+        return out
+
+    def set_topology(self, topo, core_amp_id=()):
+        topo = _u32(topo)
+        core = _u32(core_amp_id) if len(core_amp_id) else None
+        self._c(self.L.mdpd_set_topology(self.h, _u(topo) if len(topo) else None,
+                                         len(topo) // 4, _u(core), len(core_amp_id)))
+
+    def set_clock(self, time, pair_seed):
+        self._c(self.L.mdpd_set_clock(self.h, int(time), int(pair_seed)))
+
+    def get_clock(self):
+        t = C.c_uint32()
+        self._c(self.L.mdpd_get_clock(self.h, C.byref(t)))
+        return t.value
+
+    def tuning(self):
+        t = MdpdTuning()
+        self._c(self.L.mdpd_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._c(self.L.mdpd_set_tuning(self.h, C.byref(t)))
+
+    # ---- phases (main.cpp:167-196)
+    def drift(self):
+        self._c(self.L.mdpd_drift_predict(self.h))
+
+    def rebuild(self):
+        self._c(self.L.mdpd_rebuild_cells(self.h))
+
+    def density(self):
+        self._c(self.L.mdpd_density(self.h))
+
+    def force(self):
+        self._c(self.L.mdpd_force(self.h))
+
+    def bonded(self):
+        v = np.zeros(3)
+        self._c(self.L.mdpd_bonded(self.h, _d(v)))
+        return v
+
+    def kick(self):
+        self._c(self.L.mdpd_kick(self.h))
+
+    def remove_cm_drift(self):
+        self._c(self.L.mdpd_remove_cm_drift(self.h))
+
+    def head_conversion(self):
+        self._c(self.L.mdpd_head_conversion(self.h))
+
+    def chem_event(self, p_thresld, eps, influ_rad, influ_hei, influ_dep, influ_grd,
+                   max_amp_num, core_ptcl_id):
+        ids = _u32(core_ptcl_id)
+        cp = MdpdChemParams(p_thresld, eps, influ_rad, influ_hei, influ_dep, influ_grd,
+                            max_amp_num, len(ids), _u(ids) if len(ids) else None)
+        stop = C.c_int(0)
+        self._c(self.L.mdpd_chem_event(self.h, C.byref(cp), C.byref(stop)))
+        return stop.value
+
+    def run(self, nsteps):
+        self._c(self.L.mdpd_run(self.h, int(nsteps)))
+
+    def sync(self):
+        self._c(self.L.mdpd_sync(self.h))
+
+    def observe(self):
+        o = MdpdObservables()
+        self._c(self.L.mdpd_observe(self.h, C.byref(o)))
+        return observables_dict(o)
+
+    def observe_replicas(self):
+        arr = (MdpdObservables * self.n_rep)()
+        self._c(self.L.mdpd_observe_replicas(self.h, arr))
+        return [observables_dict(o) for o in arr]
+
+    def density_profile(self, axis, nbins):
+        out = np.zeros((self.n_rep, nbins))
+        self._c(self.L.mdpd_density_profile(self.h, axis, nbins, _d(out)))
+        return out
+
+    def pairs(self, cutoff=1.0):
+        n = C.c_uint64()
+        self._c(self.L.mdpd_neighbor_pairs(self.h, cutoff, None, None, 0, C.byref(n)))
+        m = n.value
+        oi = np.zeros(max(m, 1), np.uint32)
+        oj = np.zeros(max(m, 1), np.uint32)
+        self._c(self.L.mdpd_neighbor_pairs(self.h, cutoff, _u(oi), _u(oj), m, C.byref(n)))
+        return oi[:m], oj[:m]
+
+    # ---- measurement
+    def set_profiling(self, on):
+        self._c(self.L.mdpd_set_profiling(self.h, int(bool(on))))
+
+    def phase_times(self):
+        ms = np.zeros(32)
+        n = C.c_int32()
+        names = C.create_string_buffer(512)
+        self._c(self.L.mdpd_phase_times(self.h, _d(ms), 32, C.byref(n), names, 512))
+        raw = names.raw.split(b"\0")
+        return {raw[k].decode(): float(ms[k]) for k in range(n.value)}
+
+    def autotune(self, nsteps=5):
+        t = MdpdTuning()
+        self._c(self.L.mdpd_autotune(self.h, int(nsteps), C.byref(t)))
+        return t
+
+
+def saru_device(s1, s2, s3):
+    s1, s2, s3 = _u32(s1), _u32(s2), _u32(s3)
+    n = len(s1)
+    u = np.zeros(8 * n, np.uint32)
+    f = np.zeros(n, np.float32)
+    check(capi.lib().mdpd_saru_device(_u(s1), _u(s2), _u(s3), n, _u(u),
+                                      f.ctypes.data_as(C.POINTER(C.c_float))))
+    return u.reshape(n, 8), f
+
+
+def read_run_param(path):
+    p = MdpdParams()
+    ri = capi.MdpdRunInput()
+    check(capi.lib().mdpd_read_run_param(str(path).encode(), C.byref(p), C.byref(ri)))
+    return p, ri
+
+
+def device_name():
+    b = C.create_string_buffer(256)
+    check(capi.lib().mdpd_device_name(b, 256))
+    return b.value.decode()

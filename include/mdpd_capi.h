@@ -1,0 +1,251 @@
+/*
+ * mdpd_capi.h -- C ABI of the MI355X MDPD engine (implicit_dpd_amd).
+ *
+ * The engine replaces the step path of kohnakagawa/implicit_dpd: the FDPS
+ * short-range functors CalcDensity / CalcForceEpEpDPD
+ * (include/nbdf_calculator.hpp:11-145), the serial bonded force
+ * ForceBonded::CalcListedForce (include/bdf_calculator.hpp:191-220), the
+ * integrator drift_and_predict / kick / remove_cmdrift_global
+ * (include/driftkick.hpp:5-54), the serial chemistry
+ * ChemManager::RandomChemEvent (include/chemmanager.hpp:244-281) and the
+ * global reductions of Observer (include/observer.hpp:82-178).  The FDPS
+ * neighbour machinery they sit on (DomainInfo, exchangeParticle,
+ * TreeForForceShort::Gather, called at src/main.cpp:111-132,169-174) becomes
+ * a single-device cell list inside the engine.
+ *
+ * Conventions: plain pointers and sizes only; every call returns an MDPD_*
+ * status (no exceptions cross the ABI); the context owns all device memory;
+ * host buffers are borrowed for the duration of a call; one host thread per
+ * context.  Calls enqueue on the context's own HIP stream; the calls that
+ * hand data to the host (download, observe, bonded virial, chemistry) wait
+ * for it.  mdpd_last_error() gives the message of the last failing call.
+ *
+ * Force law (per ordered pair i != j with r = |r_i - r_j| < 1, r^ = r_ij/r),
+ * the generalisation of nbdf_calculator.hpp:108-132:
+ *   F_ij = [ (a0 + c1*max(r-arc,0)) * (1-r)
+ *          + sum_k mb[k] (rho_i^k + rho_j^k) * max(1 - r/rdm, 0)
+ *          + sigma * wR(r) * xi_ij
+ *          - gamma * wD(r) * (r^ . v_ij) ] r^
+ *   rho_i^k = sum_{j != i, r_ij < rd, r_ij > 0} crho (1 - r_ij/rd)^2,
+ *             k = type of j (n_dens == 3) or k = 0 (n_dens == 1)
+ *   wR = sqrt(1-r), wD = 1-r  (envelope 0: nbdf_calculator.hpp:125-127)
+ *   wR = 1-r,       wD = (1-r)^2 (envelope 1: Groot-Warren)
+ *   xi_ij = Saru(min(id), max(id), pair_seed + time).nrml()
+ *           (nbdf_calculator.hpp:99-105, saruprng.hpp:239-255,427-445)
+ * The reference model is a0 = 0, c1 = cf_c, arc = 0.9, mb = cf_m, rd = rdm
+ * = crho = 1, n_dens = 3 (parameter.hpp:138-186); Warren MDPD is a0 = A,
+ * c1 = 0, mb = B, rdm = rd, crho = 15/(2 pi rd^3), n_dens = 1; the
+ * PAIRWISE_DPD build is a0 = 25, c1 = mb = 0 (nbdf_calculator.hpp:114-117).
+ */
+#ifndef MDPD_CAPI_H
+#define MDPD_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDPD_MAX_SPECIES 3
+/* sizeof(FPDPD) in include/ptcl_class.hpp:29-36 (u32 x4, F64vec x8, F64 x3) */
+#define MDPD_FPDPD_STRIDE 232
+#define MDPD_UNIT_NONE 0xffffffffu
+
+/* status codes */
+enum {
+  MDPD_OK = 0,
+  MDPD_ERR_ARG = 1,           /* bad argument (null, size, range) */
+  MDPD_ERR_HIP = 2,           /* a HIP runtime call failed */
+  MDPD_ERR_CAPACITY = 3,      /* more beads than the context holds */
+  MDPD_ERR_NONFINITE = 4,     /* a non-finite position/velocity/force */
+  MDPD_ERR_OUT_OF_BOX = 5,    /* a bead outside [0, L] (parameter.hpp:454-460) */
+  MDPD_ERR_NLIST_OVERFLOW = 6,/* more neighbours than the pair-list width */
+  MDPD_ERR_PARSE = 7,         /* run_param.txt / xyz parse error */
+  MDPD_ERR_STATE = 8,         /* call out of order (e.g. force before density) */
+  MDPD_ERR_NO_DEVICE = 9,     /* no HIP device */
+  MDPD_ERR_UNSUPPORTED = 10   /* option not available for this context */
+};
+
+/* bead types, parameter.hpp:211-217 */
+enum { MDPD_HYPHIL = 0, MDPD_HYPHOB = 1, MDPD_SOLVENT = 2 };
+
+/* Model and run constants of one system (one replica). */
+typedef struct mdpd_params {
+  double box[3];            /* box_leng (parameter.hpp:75) */
+  double dt;                /* dt */
+  int32_t n_species;        /* bead types present, 1..3 */
+  int32_t n_dens;           /* density slots: 3 (per neighbour type) or 1 */
+  int32_t envelope;         /* 0: reference sqrt envelope, 1: Groot-Warren */
+  int32_t pad0;
+  double rd;                /* density cutoff (1 in the reference) */
+  double crho;              /* density weight normalisation */
+  double rdm;               /* many-body envelope cutoff */
+  double arc;               /* onset of the c1 term (Parameter::arc = 0.9) */
+  double a0[3][3];          /* constant conservative amplitude */
+  double c1[3][3];          /* cf_c */
+  double mb[3][3][3];       /* cf_m */
+  double sigma[3][3];       /* cf_r, already divided by sqrt(dt) */
+  double gamma[3][3];       /* cf_g */
+  /* bonded chains, bdf_calculator.hpp:95-150 */
+  double cf_s;              /* spring: F = cf_s (1/r - 1/bond_leng) dr */
+  double cf_b;              /* bending U = cf_b (1 - cos) */
+  double cf_b_rigid;        /* bending of core amphiphiles */
+  double bond_leng;         /* Parameter::bond_leng = 0.5 */
+  /* random streams */
+  uint32_t pair_seed;       /* CalcForceEpEpDPD::m_seed */
+  uint32_t chem_seed;       /* keyed draws of the chemistry */
+  /* head-conversion reaction: a unit-0 bead of type conv_from becomes
+     conv_to with probability conv_p per step (no reference counterpart) */
+  double conv_p;
+  int32_t conv_from;
+  int32_t conv_to;
+  uint32_t cm_corr;         /* steps between CM-drift removals (100) */
+  uint32_t pad1;
+} mdpd_params;
+
+/* Chemistry of the serial reference (chemmanager.hpp:116-173,244-281). */
+typedef struct mdpd_chem_params {
+  double p_thresld;         /* insertion probability per candidate */
+  double eps;               /* tangential offset of the copy */
+  double influ_rad, influ_hei, influ_dep, influ_grd; /* truncated cone */
+  uint32_t max_amp_num;     /* stop once amp_num >= max_amp_num */
+  uint32_t n_core;          /* number of core amphiphiles */
+  const uint32_t* core_ptcl_id; /* bead ids of the core heads [n_core] */
+} mdpd_chem_params;
+
+/* Global reductions of Observer (observer.hpp:82-178), FP64. */
+typedef struct mdpd_observables {
+  double kin_temp[4];       /* <vx^2>, <vy^2>, <vz^2>, mean  (KineticTempera) */
+  double pressure[3];       /* P_xx, P_yy, P_zz                (Pressure) */
+  double diffusion[3];      /* MSD per bead, per amphiphile CM, per solvent */
+  double bonded_vir[3];     /* twice the bonded virial (bdf_calculator.hpp:105-142) */
+  double pair_vir[3];       /* sum_i press_i (twice the pair virial) */
+  uint32_t n_beads;
+  uint32_t n_amp;
+  uint32_t time;
+  uint32_t pad;
+} mdpd_observables;
+
+/* Launch choices the autotuner picks; results do not depend on them. */
+typedef struct mdpd_tuning {
+  int32_t block_scan;       /* threads per block, density/pair-list pass */
+  int32_t block_force;      /* threads per block, force pass */
+  int32_t cell_div;         /* cells per unit cutoff along each axis (1 or 2) */
+  int32_t nlist_width;      /* pair-list slots per bead */
+  int32_t reorder_every;    /* steps between spatial reorders of bead storage (0: never) */
+  int32_t use_graph;        /* capture each step in a hipGraph */
+} mdpd_tuning;
+
+typedef struct mdpd_ctx mdpd_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* One system (FDPS ParticleSystem + DomainInfo + two trees, main.cpp:93-127). */
+int mdpd_create(const mdpd_params* p, uint32_t capacity, mdpd_ctx** out);
+/* n_rep independent replicas in one context; replica r holds n_beads[r]
+   beads, uploaded back to back.  No reference counterpart: the reference
+   runs one system per process (test/simple_dpdfluid/run_all.sh:5-13). */
+int mdpd_create_ensemble(const mdpd_params* p, const uint32_t* n_beads,
+                         int32_t n_rep, mdpd_ctx** out);
+int mdpd_destroy(mdpd_ctx* ctx);
+const char* mdpd_last_error(const mdpd_ctx* ctx);
+/* message of the last failure that had no context (create) */
+const char* mdpd_global_error(void);
+int mdpd_device_name(char* buf, size_t len);
+
+/* ---- state -------------------------------------------------------------- */
+/* FPDPD records (ptcl_class.hpp:29-36), `stride` bytes apart (>= 232). */
+int mdpd_upload_fpdpd(mdpd_ctx* ctx, const void* aos, uint32_t n, size_t stride);
+int mdpd_download_fpdpd(mdpd_ctx* ctx, void* aos, uint32_t n, size_t stride);
+/* Structure-of-arrays transfer; any pointer may be NULL on download.  vec3
+   arrays are [n][3] doubles; dens is [n][3]. */
+int mdpd_upload_soa(mdpd_ctx* ctx, uint32_t n, const double* pos,
+                    const double* vel, const double* acc, const uint32_t* id,
+                    const uint32_t* prop, const uint32_t* amp_id,
+                    const uint32_t* unit);
+int mdpd_download_soa(mdpd_ctx* ctx, uint32_t n, double* pos, double* vel,
+                      double* vel_buf, double* acc, double* press,
+                      double* dens, double* delta_sumr, uint32_t* id,
+                      uint32_t* prop, uint32_t* amp_id, uint32_t* unit);
+int mdpd_num_beads(const mdpd_ctx* ctx, uint32_t* n);
+/* glob_topol_[amp*4+unit] = bead index (bdf_calculator.hpp:61-88); the
+   bending constant of amphiphile a is cf_b_rigid if a is listed in
+   core_amp_id, else cf_b.  n_amp may be 0 (no chains). */
+int mdpd_set_topology(mdpd_ctx* ctx, const uint32_t* glob_topol,
+                      uint32_t n_amp, const uint32_t* core_amp_id,
+                      uint32_t n_core);
+/* Parameter::time and CalcForceEpEpDPD::m_seed */
+int mdpd_set_clock(mdpd_ctx* ctx, uint32_t time, uint32_t pair_seed);
+int mdpd_get_clock(const mdpd_ctx* ctx, uint32_t* time);
+int mdpd_set_tuning(mdpd_ctx* ctx, const mdpd_tuning* t);
+int mdpd_get_tuning(const mdpd_ctx* ctx, mdpd_tuning* t);
+
+/* ---- one step, in the order of src/main.cpp:167-196 --------------------- */
+int mdpd_drift_predict(mdpd_ctx* ctx);          /* driftkick.hpp:5-30 */
+int mdpd_rebuild_cells(mdpd_ctx* ctx);          /* exchangeParticle + tree */
+int mdpd_density(mdpd_ctx* ctx);                /* CalcDensity */
+int mdpd_force(mdpd_ctx* ctx);                  /* CalcForceEpEpDPD */
+int mdpd_bonded(mdpd_ctx* ctx, double vir[3]);  /* ForceBonded::CalcListedForce */
+int mdpd_kick(mdpd_ctx* ctx);                   /* driftkick.hpp:32-40 */
+int mdpd_remove_cm_drift(mdpd_ctx* ctx);        /* driftkick.hpp:42-54 */
+int mdpd_head_conversion(mdpd_ctx* ctx);        /* keyed type flip, no counterpart */
+/* RandomChemEvent (chemmanager.hpp:244-281) with keyed draws and the new
+   amphiphiles appended to the bonded topology; *stop = 1 once
+   amp_num >= max_amp_num. */
+int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
+int mdpd_observe(mdpd_ctx* ctx, mdpd_observables* out);
+/* observables of every replica of an ensemble context, out[n_rep] */
+int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
+/* bead-count profile along `axis` in nbins bins for every replica,
+   out[n_rep][nbins] (counts per bin, FP64) */
+int mdpd_density_profile(mdpd_ctx* ctx, int32_t axis, int32_t nbins,
+                         double* out);
+/* `nsteps` iterations of main.cpp:166-196 (drift .. time++); the pair
+   virial of the last step is kept for mdpd_observe. */
+int mdpd_run(mdpd_ctx* ctx, uint32_t nsteps);
+/* Blocks until the context's stream is idle. */
+int mdpd_sync(mdpd_ctx* ctx);
+
+/* ---- checks and measurement -------------------------------------------- */
+/* Ordered neighbour pairs (i != j, r^2 < cutoff^2, i and j bead indices)
+   seen by the last rebuild, sorted by (i, j).  cutoff must be 1 or the
+   density cutoff.  Mirrors test/nearlist_gpu/near_list.cu:70-138. */
+int mdpd_neighbor_pairs(mdpd_ctx* ctx, double cutoff, uint32_t* out_i,
+                        uint32_t* out_j, uint64_t cap, uint64_t* n_out);
+/* Saru on the device: out_u32[8*k] = first 8 u32() of Saru(s1,s2,s3)[k],
+   out_nrml[k] = FP32 nrml() of a fresh stream (saruprng.hpp:239-255,
+   322-334,427-445). */
+int mdpd_saru_device(const uint32_t* s1, const uint32_t* s2,
+                     const uint32_t* s3, uint32_t n, uint32_t* out_u32,
+                     float* out_nrml);
+/* Per-phase average kernel time (ms) over the steps run since the last
+   call with profiling on; names are "drift", "sort", ..., NUL separated. */
+int mdpd_set_profiling(mdpd_ctx* ctx, int32_t on);
+int mdpd_phase_times(mdpd_ctx* ctx, double* ms, int32_t cap, int32_t* n,
+                     char* names, size_t names_len);
+/* Times candidate launch choices on the current state (nsteps each) and
+   keeps the fastest; results are bit-identical across candidates except
+   for FP32 summation order.  out may be NULL. */
+int mdpd_autotune(mdpd_ctx* ctx, uint32_t nsteps, mdpd_tuning* out);
+
+/* ---- host-side parameter input (parameter.hpp:74-186,352-387) ---------- */
+/* Reads run_param.txt in `dir`; fills p with the reference coefficients of
+   CalcInterCoef and cp/run with the run and chemistry constants.  Values
+   are parsed with std::stof like the reference (parameter.hpp:332-340). */
+typedef struct mdpd_run_input {
+  uint32_t init_amp_num, sol_num;
+  uint32_t all_time, step_mic, step_mac;
+  uint32_t max_amp_num;
+  double p_thresld, eps;
+  double influ_rad, influ_hei, influ_dep, influ_grd, influ_vol, upside_rad;
+  double chi, kappa, rho_co, vbb;
+  double cf_r_in[6];
+  uint32_t n_core;
+  uint32_t core_amp_id[64];
+} mdpd_run_input;
+int mdpd_read_run_param(const char* dir, mdpd_params* p, mdpd_run_input* in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDPD_CAPI_H */
