@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Headline benchmark: MDPD bead-steps/s at 1M beads on one GPU.
+
+Workload: the implicit-solvent amphiphile membrane system, 262,144 H1T3 chains
+(1,048,576 beads) in a 100x100x40 box, Warren MDPD with A_ij in [-40,-20] by
+bead type, B = 25, rd = 0.75, harmonic bonds k = 100 l0 = 0.5, bending 10,
+head conversion p = 1e-4 per step, dt = 0.005 (synthetic seeded random
+chains).  One step is the reference's main-loop body (src/main.cpp:167-196):
+drift, cell-list rebuild, density pass, force pass, bonded forces, kick,
+centre-of-mass drift removal every 100 steps, head conversion, clock.
+
+The timed region runs K steps with HIP events recorded on the engine stream
+around every phase; `value` is beads * K / (GPU time of the region).  The
+per-kernel averages of the same region give the force-pass roofline.  The CPU
+baseline is the FP64 oracle (the reference's per-bead gather loops, OpenMP)
+timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MDPD bead-steps/s at 1M beads, % HBM roofline; surrogate vs plain calib runs"
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # CUs x lanes/clk x clock (non-packed FP32/INT32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def force_bytes_per_launch(n, n_pairs, nd):
+    """Algorithmic HBM bytes of one force pass: per bead its own sorted
+    position (16) and velocity (16), densities (4 nd), pair count (4), slot
+    map (4), the acceleration written (12); per pair-list entry 4 bytes.
+    Neighbour data reuse is on-chip and not counted."""
+    return n * (16 + 16 + 4 * nd + 4 + 4 + 12) + 4.0 * n_pairs
+
+
+def step_bytes_per_bead(nd, bonded, rho_mean):
+    """Algorithmic HBM bytes per bead-step of the whole step (FP64 master
+    pos/vel read + written by drift and kick, FP32 force, ids and types,
+    sorted FP32 copies written once and read by both pair passes, slot map,
+    densities, bonded gather, cell offsets)."""
+    return 264 + 8 * nd + (16 if bonded else 0) + 16.0 / rho_mean
+
+
+def cpu_baseline(p, s, topo, budget_s, max_steps):
+    from oracle import pyoracle as po
+    n = len(s["pos"])
+    o = po.System(p, n, max(len(topo) // 4, 1))
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    if len(topo):
+        o.set_topology(topo)
+    o.set_clock(0)
+    o.run(1)  # warm-up step (page faults, thread start)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        o.run(1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "bead-steps/s", "cores": po.num_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle (per-bead gather loops, OpenMP) on the same {n}-bead "
+                      f"system, {steps} steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--chains", type=int, default=262144)
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--calib-bytes", type=float, default=0.0,
+                    help="only run the copy-kernel probe of this many bytes (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from implicit_dpd_amd import configs
+    from implicit_dpd_amd.engine import Engine, bandwidth_probe, device_name
+
+    if args.calib_bytes:
+        g = bandwidth_probe(int(args.calib_bytes), 10)
+        print(json.dumps({"copy_probe_bytes": int(args.calib_bytes), "reps": 11,
+                          "gbps": g}), flush=True)
+        return
+
+    p, s = configs.membrane(n_chain=args.chains)
+    n = len(s["pos"])
+    box = [p.box[0], p.box[1], p.box[2]]
+    rho_mean = n / (box[0] * box[1] * box[2])
+    log(f"[bench] device {device_name()}; {n} beads, box {box}")
+    e = Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(0, p.pair_seed)
+    t = time.perf_counter()
+    e.run(max(args.warmup, 1))
+    e.sync()
+    log(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t:.2f} s")
+    if args.autotune:
+        tt = e.autotune(10)
+        log(f"[bench] autotune: block_scan {tt.block_scan}, block_force {tt.block_force}")
+
+    # timed region: K steps, HIP events around every phase on the engine stream
+    e.set_profiling(True)
+    t = time.perf_counter()
+    e.run(args.steps)
+    e.sync()
+    wall = time.perf_counter() - t
+    ph = e.phase_times()
+    e.set_profiling(False)
+    ms_step = ph["step"]
+    value = n / (ms_step * 1e-3)
+    log(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step (events), wall {wall:.3f} s")
+    log("[bench] phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in ph.items()))
+
+    # the same K steps replayed as captured graphs (production mode), for reference
+    t = time.perf_counter()
+    e.run(args.steps)
+    e.sync()
+    graph_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    obs = e.observe()
+    n_pairs = obs["n_pairs"]
+    nd = p.n_dens
+
+    fb = force_bytes_per_launch(n, n_pairs, nd)
+    f_ms = ph["force"]
+    achieved = fb / (f_ms * 1e-3) / 1e9
+    bstep = step_bytes_per_bead(nd, True, rho_mean)
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "k_force (force pass)",
+        "kernel_ms": round(f_ms, 5),
+        "algorithmic_bytes_per_launch": int(fb),
+        "pairs_per_bead": round(n_pairs / n, 3),
+        "binding_roof": "VALU (FP32/INT32): per pair Saru hash + log/sqrt/cos; see valu_* from rocprof",
+    }
+    step_hbm = {"bytes_per_bead_step": round(bstep, 1),
+                "achieved_gbs": round(value * bstep / 1e9, 1),
+                "frac": round(value * bstep / (HBM_PEAK_GBS * 1e9), 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)...")
+        cpu = cpu_baseline(p, s, s["topo"], args.cpu_seconds, 400)
+        log(f"[bench] CPU: {cpu['value']:.4g} bead-steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "membrane: 262144 H1T3 chains (1048576 beads), box 100x100x40, Warren "
+                        "MDPD A_ij in [-40,-20] by type, B=25, rd=0.75, bonds k=100 l0=0.5, "
+                        "bending 10, head conversion p=1e-4/step, dt=0.005",
+            "beads": n,
+            "precision": "pair arithmetic FP32; integration, wrap, reductions FP64",
+            "graph_ms_per_step": round(graph_ms, 5),
+            "phase_ms": {k: round(v, 5) for k, v in ph.items()},
+        },
+        "roofline": roof,
+        "step_hbm_roofline": step_hbm,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    e.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,7 @@ class MdpdObservables(C.Structure):
         ("diffusion", C.c_double * 3),
         ("bonded_vir", C.c_double * 3),
         ("pair_vir", C.c_double * 3),
+        ("n_pairs", C.c_double),
         ("n_beads", C.c_uint32),
         ("n_amp", C.c_uint32),
         ("time", C.c_uint32),
@@ -171,6 +172,7 @@ PROTOTYPES = {
     "mdpd_set_profiling": (C.c_int, [VP, C.c_int32]),
     "mdpd_phase_times": (C.c_int, [VP, _DP, C.c_int32, P(C.c_int32), C.c_char_p, C.c_size_t]),
     "mdpd_autotune": (C.c_int, [VP, C.c_uint32, P(MdpdTuning)]),
+    "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
 }
 

@@ -38,8 +38,10 @@ struct mdpd_ctx {
   bool drifted = false, have_cells = false, cells_fresh = false, have_density = false;
   hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   bool profiling = false;
-  hipEvent_t ev[mdpd::kPhases + 1] = {};
-  double ph_ms[mdpd::kPhases] = {0};
+  // profiling: kPhases + 1 events per profiled step, read back lazily
+  std::vector<hipEvent_t> evpool;
+  uint32_t ev_steps = 0;          // profiled steps recorded in evpool
+  double ph_ms[mdpd::kPhases + 1] = {0};  // accumulated; last = whole step
   uint64_t ph_steps = 0;
   std::vector<void*> allocs;
 };

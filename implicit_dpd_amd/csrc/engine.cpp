@@ -240,7 +240,6 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
     mdpd_destroy(c);
     return fail(nullptr, MDPD_ERR_HIP, "initial upload failed");
   }
-  for (auto& e : c->ev) hipEventCreate(&e);
   *out = c;
   return MDPD_OK;
 }
@@ -250,8 +249,19 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
   const DevState& d = c->d;
   hipStream_t st = c->st;
   const bool prof = c->profiling;
+  hipEvent_t* ev = nullptr;
+  if (prof) {
+    const size_t need = (size_t)(c->ev_steps + 1) * (kPhases + 1);
+    while (c->evpool.size() < need) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      c->evpool.push_back(e);
+    }
+    ev = &c->evpool[(size_t)c->ev_steps * (kPhases + 1)];
+    c->ev_steps++;
+  }
   auto mark = [&](int k) {
-    if (prof) hipEventRecord(c->ev[k], st);
+    if (prof) hipEventRecord(ev[k], st);
   };
   mark(0);
   launch_fill_u32(d.cell_cnt, d.ncell + 1, 0u, st);
@@ -314,15 +324,6 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
       HIPCHK(c, hipGraphLaunch(c->gexec[v], c->st));
     } else {
       enqueue_step(c, virial, cm);
-      if (c->profiling) {
-        HIPCHK(c, hipEventSynchronize(c->ev[kPhases]));
-        for (int p = 0; p < kPhases; ++p) {
-          float ms = 0.f;
-          hipEventElapsedTime(&ms, c->ev[p], c->ev[p + 1]);
-          c->ph_ms[p] += ms;
-        }
-        c->ph_steps++;
-      }
     }
     c->time++;
   }
@@ -382,8 +383,7 @@ int mdpd_destroy(mdpd_ctx* c) {
   if (!c) return MDPD_OK;
   if (c->st) hipStreamSynchronize(c->st);
   invalidate_graphs(c);
-  for (auto& e : c->ev)
-    if (e) hipEventDestroy(e);
+  for (auto& e : c->evpool) hipEventDestroy(e);
   for (void* q : c->allocs) hipFree(q);
   if (c->pinned) hipHostFree(c->pinned);
   if (c->st) hipStreamDestroy(c->st);
@@ -781,7 +781,7 @@ static int observe_impl(mdpd_ctx* c, mdpd_observables* out, int all) {
   if (!c || !out) return MDPD_ERR_ARG;
   const DevState& d = c->d;
   launch_observe(d, c->d_amp_num, c->obs_out, c->st);
-  std::vector<double> q((size_t)c->n_rep * 12), bv((size_t)c->n_rep * 3);
+  std::vector<double> q((size_t)c->n_rep * 13), bv((size_t)c->n_rep * 3);
   HIPCHK(c, hipMemcpyAsync(q.data(), c->obs_out, sizeof(double) * q.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipMemcpyAsync(bv.data(), c->vir_bonded, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -789,7 +789,8 @@ static int observe_impl(mdpd_ctx* c, mdpd_observables* out, int all) {
   for (int r = 0; r < nr; ++r) {
     mdpd_observables& o = out[r];
     memset(&o, 0, sizeof(o));
-    const double* s = &q[(size_t)r * 12];
+    const double* s = &q[(size_t)r * 13];
+    o.n_pairs = s[12];
     const uint32_t n_r = c->rep_off[r + 1] - c->rep_off[r];
     for (int a = 0; a < 3; ++a) o.kin_temp[a] = s[a] / n_r;
     o.kin_temp[3] = (o.kin_temp[0] + o.kin_temp[1] + o.kin_temp[2]) / 3.0;
@@ -904,29 +905,82 @@ int mdpd_saru_device(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3,
   return MDPD_OK;
 }
 
+// fold the recorded per-step events into the per-phase sums
+static int collect_events(mdpd_ctx* c) {
+  if (!c->ev_steps) return MDPD_OK;
+  HIPCHK(c, hipEventSynchronize(c->evpool[(size_t)c->ev_steps * (kPhases + 1) - 1]));
+  for (uint32_t k = 0; k < c->ev_steps; ++k) {
+    hipEvent_t* ev = &c->evpool[(size_t)k * (kPhases + 1)];
+    for (int p = 0; p < kPhases; ++p) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[p], ev[p + 1]));
+      c->ph_ms[p] += ms;
+    }
+  }
+  // whole-region time: first mark of the first step to the last mark of the last
+  float span = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&span, c->evpool[0],
+                                c->evpool[(size_t)c->ev_steps * (kPhases + 1) - 1]));
+  c->ph_ms[kPhases] += span;
+  c->ph_steps += c->ev_steps;
+  c->ev_steps = 0;
+  return MDPD_OK;
+}
+
 int mdpd_set_profiling(mdpd_ctx* c, int32_t on) {
   if (!c) return MDPD_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
   c->profiling = on != 0;
   for (double& x : c->ph_ms) x = 0.0;
   c->ph_steps = 0;
+  c->ev_steps = 0;
   return MDPD_OK;
 }
 
 int mdpd_phase_times(mdpd_ctx* c, double* ms, int32_t cap, int32_t* n, char* names, size_t names_len) {
   if (!c || !n) return MDPD_ERR_ARG;
-  *n = kPhases;
-  for (int p = 0; p < kPhases && p < cap; ++p) ms[p] = c->ph_steps ? c->ph_ms[p] / c->ph_steps : 0.0;
+  int rc = collect_events(c);
+  if (rc) return rc;
+  *n = kPhases + 1;
+  for (int p = 0; p <= kPhases && p < cap; ++p) ms[p] = c->ph_steps ? c->ph_ms[p] / c->ph_steps : 0.0;
   if (names && names_len) {
     size_t off = 0;
-    for (int p = 0; p < kPhases; ++p) {
-      const size_t l = strlen(kPhaseNames[p]);
+    for (int p = 0; p <= kPhases; ++p) {
+      const char* nm = p < kPhases ? kPhaseNames[p] : "step";
+      const size_t l = strlen(nm);
       if (off + l + 1 >= names_len) break;
-      memcpy(names + off, kPhaseNames[p], l);
+      memcpy(names + off, nm, l);
       names[off + l] = '\0';
       off += l + 1;
     }
     if (off < names_len) names[off] = '\0';
   }
+  return MDPD_OK;
+}
+
+int mdpd_bandwidth_probe(uint64_t bytes, int32_t reps, double* gbps) {
+  if (!gbps || bytes < 1024 || reps < 1) return fail(nullptr, MDPD_ERR_ARG, "bad arguments");
+  double *a = nullptr, *b = nullptr;
+  const size_t nd = bytes / 8;
+  if (hipMalloc(&a, nd * 8) != hipSuccess || hipMalloc(&b, nd * 8) != hipSuccess)
+    return fail(nullptr, MDPD_ERR_HIP, "hipMalloc failed");
+  hipMemset(a, 0, nd * 8);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  launch_copy_d(b, a, nd, nullptr);
+  hipEventRecord(e0, nullptr);
+  for (int k = 0; k < reps; ++k) launch_copy_d(b, a, nd, nullptr);
+  hipEventRecord(e1, nullptr);
+  const hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(nullptr, MDPD_ERR_HIP, hipGetErrorString(e));
+  *gbps = 2.0 * (double)nd * 8.0 * reps / (ms * 1e-3) / 1e9;
   return MDPD_OK;
 }
 

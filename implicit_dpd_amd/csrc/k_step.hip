@@ -381,7 +381,8 @@ void launch_cm_drift(const DevState& d, hipStream_t st) {
 //  0-2 sum v_a^2; 3-5 sum (0.5 press_a + v_a^2); 6-8 sum press_a;
 //  9 sum |delta|^2; 10 sum |delta|^2 over local index >= amp_num (the
 //  reference's solvent loop, observer.hpp:171); 11 sum over amphiphiles of
-//  |mean delta of beads 4a..4a+3|^2 (observer.hpp:160-167).
+//  |mean delta of beads 4a..4a+3|^2 (observer.hpp:160-167); 12 number of
+//  pair-list entries (sorted slots of the replica).
 __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
                                                          const uint32_t* __restrict__ amp_num,
                                                          double* __restrict__ part) {
@@ -390,9 +391,9 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
   const uint32_t b0 = d.rep_off[r], n_r = d.rep_off[r + 1] - b0;
   const uint32_t na = amp_num[r];
   const uint32_t cap = d.cap;
-  KSum acc[12];
+  KSum acc[13];
 #pragma unroll
-  for (int q = 0; q < 12; ++q) acc[q] = KSum{0.0, 0.0};
+  for (int q = 0; q < 13; ++q) acc[q] = KSum{0.0, 0.0};
   for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < n_r; k += kRedBlocks * kRedThreads) {
     const uint32_t i = b0 + k;
     double dd = 0.0;
@@ -407,6 +408,7 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
       dd += dl * dl;
     }
     acc[9].add(dd);
+    acc[12].add((double)min(d.ncount[i], (uint32_t)d.W));
     if (k >= na) acc[10].add(dd);
     if ((k & 3u) == 0 && (k >> 2) < na && k + 3 < n_r) {
       double m[3];
@@ -420,13 +422,13 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
       acc[11].add(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
     }
   }
-  block_reduce_write<12>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 2 * 12);
+  block_reduce_write<13>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 2 * 13);
 }
 
 void launch_observe(const DevState& d, const uint32_t* amp_num, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_observe, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d, amp_num,
                      d.red);
-  hipLaunchKernelGGL(k_red_final<12>, dim3(d.n_rep), dim3(64), 0, st, d.red, out);
+  hipLaunchKernelGGL(k_red_final<13>, dim3(d.n_rep), dim3(64), 0, st, d.red, out);
 }
 
 // ---------------------------------------------------------------- bonded

@@ -40,6 +40,7 @@ def observables_dict(o):
         "diffusion": np.array(o.diffusion[:]),
         "bonded_vir": np.array(o.bonded_vir[:]),
         "pair_vir": np.array(o.pair_vir[:]),
+        "n_pairs": float(o.n_pairs),
         "n_beads": o.n_beads,
         "n_amp": o.n_amp,
         "time": o.time,
@@ -236,6 +237,12 @@ def saru_device(s1, s2, s3):
     check(capi.lib().mdpd_saru_device(_u(s1), _u(s2), _u(s3), n, _u(u),
                                       f.ctypes.data_as(C.POINTER(C.c_float))))
     return u.reshape(n, 8), f
+
+
+def bandwidth_probe(nbytes, reps=20):
+    g = C.c_double()
+    check(capi.lib().mdpd_bandwidth_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def read_run_param(path):

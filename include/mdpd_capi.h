@@ -121,6 +121,7 @@ typedef struct mdpd_observables {
   double diffusion[3];      /* MSD per bead, per amphiphile CM, per solvent */
   double bonded_vir[3];     /* twice the bonded virial (bdf_calculator.hpp:105-142) */
   double pair_vir[3];       /* sum_i press_i (twice the pair virial) */
+  double n_pairs;           /* ordered pairs inside the cutoff, last pair list */
   uint32_t n_beads;
   uint32_t n_amp;
   uint32_t time;
@@ -223,6 +224,10 @@ int mdpd_saru_device(const uint32_t* s1, const uint32_t* s2,
 int mdpd_set_profiling(mdpd_ctx* ctx, int32_t on);
 int mdpd_phase_times(mdpd_ctx* ctx, double* ms, int32_t cap, int32_t* n,
                      char* names, size_t names_len);
+/* Device copy of `bytes` bytes (read + write), `reps` times, with the
+   engine's own copy kernel; *gbps = 2 * bytes * reps / time.  Calibrates
+   HBM counters and confirms the bandwidth of the device. */
+int mdpd_bandwidth_probe(uint64_t bytes, int32_t reps, double* gbps);
 /* Times candidate launch choices on the current state (nsteps each) and
    keeps the fastest; results are bit-identical across candidates except
    for FP32 summation order.  out may be NULL. */

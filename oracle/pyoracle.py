@@ -111,6 +111,10 @@ def saru_batch(s1, s2, s3):
     return o_u.reshape(n, 8), o_uni.reshape(n, 2), o_n
 
 
+def pair_nrml(id_i, id_j, seed3):
+    return lib().oracle_pair_nrml(int(id_i), int(id_j), int(seed3))
+
+
 def calc_inter_coef(chi, kappa, rho_co, vbb, cf_r6, dt, reo=1.4, all_unit=4, arc=0.9):
     p = MdpdParams()
     c6 = f64(cf_r6)
