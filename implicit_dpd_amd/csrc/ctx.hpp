@@ -36,7 +36,11 @@ struct mdpd_ctx {
   size_t pinned_bytes = 0;
   uint32_t time = 0;
   bool drifted = false, have_cells = false, cells_fresh = false, have_density = false;
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // captured steps by (storage parity, virial, cm-drift step)
+  hipGraphExec_t gexec[2][4] = {{nullptr, nullptr, nullptr, nullptr},
+                                {nullptr, nullptr, nullptr, nullptr}};
+  int parity = 0;            // which buffer set is current (flips every rebuild)
+  uint32_t* inv_orig = nullptr;  // scratch [cap]
   bool profiling = false;
   // profiling: kPhases + 1 events per profiled step, read back lazily
   std::vector<hipEvent_t> evpool;
@@ -49,6 +53,7 @@ struct mdpd_ctx {
 namespace mdpd {
 int fail(mdpd_ctx* c, int code, const std::string& msg);
 void invalidate_graphs(mdpd_ctx* c);
+void swap_master(mdpd_ctx* c);
 int run_steps(mdpd_ctx* c, uint32_t nsteps);
 int check_flags(mdpd_ctx* c);
 }  // namespace mdpd

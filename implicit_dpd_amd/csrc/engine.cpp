@@ -17,7 +17,7 @@ using namespace mdpd;
 namespace mdpd {
 thread_local std::string g_err;
 const char* kPhaseNames[kPhases] = {"drift", "sort", "gather", "density", "force",
-                                    "bonded", "kick", "cm_drift", "conversion", "clock"};
+                                    "bonded_kick", "kick", "cm_drift", "conversion", "clock"};
 
 int fail(mdpd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -26,11 +26,20 @@ int fail(mdpd_ctx* c, int code, const std::string& msg) {
 }
 
 void invalidate_graphs(mdpd_ctx* c) {
-  for (auto& g : c->gexec)
-    if (g) {
-      hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto& row : c->gexec)
+    for (auto& g : row)
+      if (g) {
+        hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+}
+
+// the permutation wrote the other buffer set: make it the current one
+void swap_master(mdpd_ctx* c) {
+  Master cur = static_cast<Master&>(c->d);
+  static_cast<Master&>(c->d) = c->d.alt;
+  c->d.alt = cur;
+  c->parity ^= 1;
 }
 }  // namespace mdpd
 
@@ -196,6 +205,22 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.prop, cap);
   ALLOC(d.amp_id, cap);
   ALLOC(d.unit, cap);
+  ALLOC(d.orig, cap);
+  ALLOC(d.alt.pos, 3 * (size_t)cap);
+  ALLOC(d.alt.vel, 3 * (size_t)cap);
+  ALLOC(d.alt.vbuf, 3 * (size_t)cap);
+  ALLOC(d.alt.delta, 3 * (size_t)cap);
+  ALLOC(d.alt.acc, 3 * (size_t)cap);
+  ALLOC(d.alt.press, 3 * (size_t)cap);
+  ALLOC(d.alt.id, cap);
+  ALLOC(d.alt.prop, cap);
+  ALLOC(d.alt.amp_id, cap);
+  ALLOC(d.alt.unit, cap);
+  ALLOC(d.alt.orig, cap);
+  ALLOC(d.chain, cap);
+  ALLOC(d.alt.chain, cap);
+  ALLOC(d.inv_perm, cap);
+  ALLOC(c->inv_orig, cap);
   ALLOC(d.key, cap);
   ALLOC(d.cell_cnt, (size_t)cells + 1);
   ALLOC(d.cell_start, (size_t)cells + 1);
@@ -217,7 +242,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.amp_off, n_rep + 1);
   ALLOC(d.d_time, 1);
   ALLOC(d.flags, 1);
-  ALLOC(d.red, (size_t)n_rep * 64 * 32 + 3 * (size_t)c->amp_cap);
+  ALLOC(d.red, (size_t)n_rep * 64 * 32 + 3 * (size_t)cap);
   ALLOC(d.red_out, (size_t)n_rep * 16);
   ALLOC(c->vir_bonded, (size_t)n_rep * 3);
   ALLOC(c->obs_out, (size_t)n_rep * 16);
@@ -271,15 +296,18 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
   launch_scatter(d, st);
   launch_cell_sort(d, st);
   mark(2);
-  launch_gather(d, d.vbuf, st);
+  // bead storage moves to the new cell order; the slots below are the new ones
+  launch_permute(d, true, false, st);
+  swap_master(c);
+  launch_topo_remap(d, st);
   mark(3);
   launch_density(d, c->ns, c->nd, c->tune.block_scan, st);
   mark(4);
   launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st);
   mark(5);
-  launch_bonded(d, virial, c->vir_bonded, st);
+  // ForceBonded::CalcListedForce and kick fused (bead-centric, see k_step.hip)
+  launch_bonded_kick(d, virial, true, c->vir_bonded, st);
   mark(6);
-  launch_kick(d, st);
   mark(7);
   if (cm) launch_cm_drift(d, st);
   mark(8);
@@ -313,15 +341,20 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
     const bool cm = (c->time % c->cm_corr) == 0;
     if (c->tune.use_graph && !c->profiling) {
       const int v = (virial ? 1 : 0) | (cm ? 2 : 0);
-      if (!c->gexec[v]) {
+      const int p = c->parity;
+      if (!c->gexec[p][v]) {
+        // capturing records the step and flips the host's buffer parity, as
+        // running it does
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         enqueue_step(c, virial, cm);
         HIPCHK(c, hipStreamEndCapture(c->st, &g));
-        HIPCHK(c, hipGraphInstantiate(&c->gexec[v], g, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0));
         hipGraphDestroy(g);
+      } else {
+        swap_master(c);
       }
-      HIPCHK(c, hipGraphLaunch(c->gexec[v], c->st));
+      HIPCHK(c, hipGraphLaunch(c->gexec[p][v], c->st));
     } else {
       enqueue_step(c, virial, cm);
     }
@@ -334,23 +367,6 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   return MDPD_OK;
 }
 
-namespace {
-void dens_to_master(mdpd_ctx* c, std::vector<double>& out /*[n][3]*/) {
-  const uint32_t n = c->d.n;
-  out.assign(3 * (size_t)n, 0.0);
-  if (!c->have_density || !n) return;
-  std::vector<uint32_t> perm(n);
-  std::vector<float> sd((size_t)c->nd * n);
-  hipMemcpyAsync(perm.data(), c->d.perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->st);
-  for (int k = 0; k < c->nd; ++k)
-    hipMemcpyAsync(sd.data() + (size_t)k * n, c->d.sdens + (size_t)k * c->d.cap, sizeof(float) * n,
-                   hipMemcpyDeviceToHost, c->st);
-  hipStreamSynchronize(c->st);
-  for (uint32_t s = 0; s < n; ++s)
-    for (int k = 0; k < c->nd; ++k) out[3 * (size_t)perm[s] + k] = sd[(size_t)k * n + s];
-}
-
-}  // namespace
 
 // ======================================================================
 // C ABI
@@ -454,6 +470,12 @@ int mdpd_upload_soa(mdpd_ctx* c, uint32_t n, const double* pos, const double* ve
   HIPCHK(c, hipMemcpyAsync(d.amp_id, hu + 2 * (size_t)n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipMemcpyAsync(d.unit, hu + 3 * (size_t)n, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  for (uint32_t i = 0; i < n; ++i) hu[i] = i;  // storage slot i holds upload slot i
+  HIPCHK(c, hipMemcpyAsync(d.orig, hu, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+  launch_fill_u32(d.chain, n, 0xffffffffu, c->st);  // chains come with set_topology
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  d.n_amp = 0;
+  c->n_amp = 0;
   // solvent count per replica (unit == none)
   for (int r = 0; r < c->n_rep; ++r) {
     uint32_t s = 0;
@@ -515,13 +537,19 @@ int mdpd_download_soa(mdpd_ctx* c, uint32_t n, double* pos, double* vel, double*
   if (n != c->d.n) return fail(c, MDPD_ERR_ARG, "n must equal the bead count");
   HIPCHK(c, hipStreamSynchronize(c->st));
   const DevState& d = c->d;
+  // storage is in cell order; orig[] puts every bead back at its upload slot
+  std::vector<uint32_t> orig(n);
+  HIPCHK(c, hipMemcpy(orig.data(), d.orig, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i)
+    if (orig[i] >= n) return fail(c, MDPD_ERR_STATE, "bead storage map is corrupt");
   std::vector<double> t(n);
   std::vector<float> f(n);
+  std::vector<uint32_t> u(n);
   auto get3d = [&](const double* src, double* dst) -> int {
     if (!dst) return MDPD_OK;
     for (int a = 0; a < 3; ++a) {
       HIPCHK(c, hipMemcpy(t.data(), src + (size_t)a * d.cap, 8 * (size_t)n, hipMemcpyDeviceToHost));
-      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)i + a] = t[i];
+      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)orig[i] + a] = t[i];
     }
     return MDPD_OK;
   };
@@ -529,8 +557,14 @@ int mdpd_download_soa(mdpd_ctx* c, uint32_t n, double* pos, double* vel, double*
     if (!dst) return MDPD_OK;
     for (int a = 0; a < 3; ++a) {
       HIPCHK(c, hipMemcpy(f.data(), src + (size_t)a * d.cap, 4 * (size_t)n, hipMemcpyDeviceToHost));
-      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)i + a] = f[i];
+      for (uint32_t i = 0; i < n; ++i) dst[3 * (size_t)orig[i] + a] = f[i];
     }
+    return MDPD_OK;
+  };
+  auto getu = [&](const uint32_t* src, uint32_t* dst) -> int {
+    if (!dst) return MDPD_OK;
+    HIPCHK(c, hipMemcpy(u.data(), src, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) dst[orig[i]] = u[i];
     return MDPD_OK;
   };
   int rc;
@@ -540,14 +574,17 @@ int mdpd_download_soa(mdpd_ctx* c, uint32_t n, double* pos, double* vel, double*
   if ((rc = get3d(d.delta, delta))) return rc;
   if ((rc = get3f(d.acc, acc))) return rc;
   if ((rc = get3f(d.press, press))) return rc;
-  if (id) HIPCHK(c, hipMemcpy(id, d.id, 4 * (size_t)n, hipMemcpyDeviceToHost));
-  if (prop) HIPCHK(c, hipMemcpy(prop, d.prop, 4 * (size_t)n, hipMemcpyDeviceToHost));
-  if (amp_id) HIPCHK(c, hipMemcpy(amp_id, d.amp_id, 4 * (size_t)n, hipMemcpyDeviceToHost));
-  if (unit) HIPCHK(c, hipMemcpy(unit, d.unit, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if ((rc = getu(d.id, id))) return rc;
+  if ((rc = getu(d.prop, prop))) return rc;
+  if ((rc = getu(d.amp_id, amp_id))) return rc;
+  if ((rc = getu(d.unit, unit))) return rc;
   if (dens) {
-    std::vector<double> dm;
-    dens_to_master(c, dm);
-    memcpy(dens, dm.data(), sizeof(double) * 3 * (size_t)n);
+    memset(dens, 0, sizeof(double) * 3 * (size_t)n);
+    if (c->have_density)
+      for (int k = 0; k < c->nd; ++k) {
+        HIPCHK(c, hipMemcpy(f.data(), d.sdens + (size_t)k * d.cap, 4 * (size_t)n, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) dens[3 * (size_t)orig[i] + k] = f[i];
+      }
   }
   return MDPD_OK;
 }
@@ -567,8 +604,10 @@ int mdpd_download_fpdpd(mdpd_ctx* c, void* aos, uint32_t n, size_t stride) {
     double* out = nullptr;
     HIPCHK(c, hipMalloc(&idx, 4 * (size_t)n));
     HIPCHK(c, hipMalloc(&out, 64 * (size_t)n));
-    std::vector<uint32_t> seq(n);
-    for (uint32_t i = 0; i < n; ++i) seq[i] = i;
+    // row k of the output is the bead uploaded at slot k
+    std::vector<uint32_t> orig(n), seq(n);
+    HIPCHK(c, hipMemcpy(orig.data(), c->d.orig, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) seq[orig[i]] = i;
     HIPCHK(c, hipMemcpy(idx, seq.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
     launch_neighbor_cm(c->d, idx, n, out, c->st);
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -623,7 +662,16 @@ int mdpd_set_topology(mdpd_ctx* c, const uint32_t* topo, uint32_t n_amp, const u
   for (int r = 0; r < c->n_rep; ++r) c->amp_off[r + 1] = c->amp_off[r] + c->amp_num_rep[r];
   DevState& d = c->d;
   if (n_amp) {
-    HIPCHK(c, hipMemcpy(d.topo, topo, 16 * (size_t)n_amp, hipMemcpyHostToDevice));
+    // the topology names upload slots; storage is in cell order
+    std::vector<uint32_t> orig(d.n), slot(d.n), tp(4 * (size_t)n_amp);
+    HIPCHK(c, hipMemcpy(orig.data(), d.orig, 4 * (size_t)d.n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < d.n; ++i) slot[orig[i]] = i;
+    for (size_t k = 0; k < tp.size(); ++k) tp[k] = slot[topo[k]];
+    HIPCHK(c, hipMemcpy(d.topo, tp.data(), 16 * (size_t)n_amp, hipMemcpyHostToDevice));
+    // each bead's chain and position in it (glob_topol_[amp * 4 + unit])
+    std::vector<uint32_t> ch(d.n, 0xffffffffu);
+    for (size_t k = 0; k < tp.size(); ++k) ch[tp[k]] = (uint32_t)((k / 4) << 2 | (k & 3));
+    HIPCHK(c, hipMemcpy(d.chain, ch.data(), 4 * (size_t)d.n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.cf_bend, cfb.data(), 8 * (size_t)n_amp, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.amp_rep, arep.data(), 4 * (size_t)n_amp, hipMemcpyHostToDevice));
   }
@@ -701,8 +749,11 @@ int mdpd_rebuild_cells(mdpd_ctx* c) {
   launch_scan_cells(d, c->st);
   launch_scatter(d, c->st);
   launch_cell_sort(d, c->st);
-  // the force uses the reference's vel: the predicted velocity after a drift
-  launch_gather(d, c->drifted ? d.vbuf : d.vel, c->st);
+  // the force uses the reference's vel: the predicted velocity after a
+  // drift; without one, vel, acc and press are still live and move along
+  launch_permute(d, c->drifted, true, c->st);
+  swap_master(c);
+  launch_topo_remap(c->d, c->st);
   c->have_cells = c->cells_fresh = true;
   c->have_density = false;
   return MDPD_OK;
@@ -726,7 +777,7 @@ int mdpd_force(mdpd_ctx* c) {
 
 int mdpd_bonded(mdpd_ctx* c, double vir[3]) {
   if (!c) return MDPD_ERR_ARG;
-  launch_bonded(c->d, true, c->vir_bonded, c->st);
+  launch_bonded_kick(c->d, true, false, c->vir_bonded, c->st);
   if (vir) {
     HIPCHK(c, hipMemcpyAsync(vir, c->vir_bonded, 24, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -780,7 +831,7 @@ int mdpd_sync(mdpd_ctx* c) {
 static int observe_impl(mdpd_ctx* c, mdpd_observables* out, int all) {
   if (!c || !out) return MDPD_ERR_ARG;
   const DevState& d = c->d;
-  launch_observe(d, c->d_amp_num, c->obs_out, c->st);
+  launch_observe(d, c->d_amp_num, c->inv_orig, c->obs_out, c->st);
   std::vector<double> q((size_t)c->n_rep * 13), bv((size_t)c->n_rep * 3);
   HIPCHK(c, hipMemcpyAsync(q.data(), c->obs_out, sizeof(double) * q.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipMemcpyAsync(bv.data(), c->vir_bonded, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, c->st));
@@ -842,7 +893,8 @@ int mdpd_neighbor_pairs(mdpd_ctx* c, double cutoff, uint32_t* out_i, uint32_t* o
     std::vector<uint32_t> nl((size_t)cap64 * W), cnt(n), perm(n);
     HIPCHK(c, hipMemcpyAsync(nl.data(), c->d.nlist, 4 * nl.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d.ncount, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(perm.data(), c->d.perm, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    // report beads by their upload slot
+    HIPCHK(c, hipMemcpyAsync(perm.data(), c->d.orig, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (uint32_t s = 0; s < n; ++s) {
       if (cnt[s] > W) return fail(c, MDPD_ERR_NLIST_OVERFLOW, "pair list overflow");

@@ -43,7 +43,7 @@ __device__ __forceinline__ double fp64_r2(const DevState& d, const Replica& R, u
 __device__ __forceinline__ bool decide(float r2, float cut2f, double cut2, const DevState& d,
                                        const Replica& R, uint32_t s, uint32_t t) {
   if (fabsf(r2 - cut2f) > kGuard) return r2 < cut2f;
-  return fp64_r2(d, R, d.perm[s], d.perm[t]) < cut2;
+  return fp64_r2(d, R, s, t) < cut2;
 }
 
 template <int ND, bool MULTI>
@@ -76,23 +76,31 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       qy += (qy < 0) ? ny : 0;
       qy -= (qy >= ny) ? ny : 0;
       const float by = pi.y - (float)oy * R.hf[1];
-      for (int ox = -1; ox <= 1; ++ox) {
-        int qx = cx + ox;
-        qx += (qx < 0) ? nx : 0;
-        qx -= (qx >= nx) ? nx : 0;
-        const float bx = pi.x - (float)ox * R.hf[0];
-        const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-        const uint32_t t0 = d.cell_start[q], t1 = d.cell_start[q + 1];
-        const uint32_t code = (uint32_t)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
+      // The three x-neighbour cells of a row are consecutive in cell order,
+      // hence one contiguous run of slots (two runs where the row wraps).
+      const uint32_t rowq = R.cell_off + (uint32_t)((qz * ny + qy) * nx);
+      const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
+      const int npiece = (cx >= 1 && cx <= nx - 2) ? 1 : 2;
+      for (int pc = 0; pc < npiece; ++pc) {
+        int xa, xb, oa;
+        if (npiece == 1) { xa = cx - 1; xb = cx + 1; oa = -1; }
+        else if (cx == 0) { xa = pc ? 0 : nx - 1; xb = pc ? 1 : nx - 1; oa = pc ? 0 : -1; }
+        else { xa = pc ? 0 : nx - 2; xb = pc ? 0 : nx - 1; oa = pc ? 1 : -1; }
+        const uint32_t t0 = d.cell_start[rowq + xa], t1 = d.cell_start[rowq + xb + 1];
+        const uint32_t b1 = (xb > xa) ? d.cell_start[rowq + xa + 1] : t1;
+        const uint32_t b2 = (xb > xa + 1) ? d.cell_start[rowq + xa + 2] : t1;
         for (uint32_t t = t0; t < t1; ++t) {
           if (t == s) continue;
+          const int ox = oa + (t >= b1 ? 1 : 0) + (t >= b2 ? 1 : 0);
+          const float bx = pi.x - (float)ox * R.hf[0];
+          const uint32_t code = cyz | ((uint32_t)(ox + 1) << kCodeShift);
           const float4 pj = d.spos[t];
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (r2 >= cut_hi) continue;
           const bool in_rc = decide(r2, 1.0f, 1.0, d, R, s, t);
           bool in_rd = decide(r2, rd2f, R.rd2, d, R, s, t);
-          if (in_rd && r2 < 1e-6f) in_rd = fp64_r2(d, R, d.perm[s], d.perm[t]) != 0.0;
+          if (in_rd && r2 < 1e-6f) in_rd = fp64_r2(d, R, s, t) != 0.0;
           if (in_rd) {
             const float rr = __builtin_amdgcn_sqrtf(r2);
             const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
@@ -192,7 +200,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d) {
     if (ENV == 0 && wc < kNearCut) {
       // sqrt(1 - r) magnifies the FP32 error of r next to the cutoff; take
       // 1 - r there from the FP64 master positions
-      const double r2d = fp64_r2(d, R, d.perm[s], d.perm[t]);
+      const double r2d = fp64_r2(d, R, s, t);
       wc = (float)fmax((1.0 - r2d) / (1.0 + sqrt(r2d)), 0.0);
     }
     float wR, wD;
@@ -215,7 +223,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d) {
       pz += fs * dz * dz;
     }
   }
-  const uint32_t i = d.perm[s];
+  const uint32_t i = s;
   d.acc[i] = fx;
   d.acc[d.cap + i] = fy;
   d.acc[2 * d.cap + i] = fz;
@@ -287,8 +295,8 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
           if (decide(r2, cut2f, cut2, d, R, s, t)) {
             const unsigned long long k = atomicAdd(counter, 1ull);
             if (k < cap) {
-              oi[k] = d.perm[s];
-              oj[k] = d.perm[t];
+              oi[k] = d.orig[s];
+              oj[k] = d.orig[t];
             }
           }
         }

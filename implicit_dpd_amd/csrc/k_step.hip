@@ -51,42 +51,66 @@ __device__ __forceinline__ uint32_t cell_of(const Replica& R, const double p[3])
          (uint32_t)c[0];
 }
 
+// Wave-aggregated "slot = base[key]++": lanes of a contiguous run holding
+// the same key (bead storage is in cell order, so runs are the rule) share a
+// single atomic issued by the run's first lane; members get consecutive
+// slots in lane order.  Inactive lanes break runs and take no slot.
+__device__ __forceinline__ uint32_t agg_inc(uint32_t* base, uint32_t key, bool active) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t kprev = __shfl_up(key, 1, 64);
+  const int aprev = __shfl_up(active ? 1 : 0, 1, 64);
+  const bool leader = active && (lane == 0 || !aprev || kprev != key);
+  const unsigned long long stop = __ballot(leader || !active);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(stop & upto);  // my run's leader (or me)
+  const unsigned long long above = stop & ~upto;
+  const int end = above ? (__ffsll((long long)above) - 1) : 64;
+  uint32_t old = 0;
+  if (leader) old = atomicAdd(&base[key], (uint32_t)(end - lane));
+  const uint32_t b = __shfl(old, start, 64);
+  return b + (uint32_t)(lane - start);
+}
+
 // drift_and_predict, driftkick.hpp:5-30.  vel is left as the last kicked
 // velocity; the predicted velocity (vel_buf == vel of the reference after
 // the drift) goes to vbuf.  With `keys` the cell key and the cell
 // histogram of the rebuild are produced in the same pass.
 __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
-  const Replica& R = d.reps[r];
-  const double dt = R.dt;
-  const uint32_t cap = d.cap;
-  double p[3];
-  bool fin = true;
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i0 < d.n;
+  const uint32_t i = active ? i0 : d.n - 1;
+  uint32_t k = 0;
+  if (active) {
+    const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+    const Replica& R = d.reps[r];
+    const double dt = R.dt;
+    const uint32_t cap = d.cap;
+    double p[3];
+    bool fin = true;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double v = d.vel[a * cap + i];
-    const double ac = (double)d.acc[a * cap + i];
-    if (!predict_only) {
-      const double dr = dt * (v + 0.5 * dt * ac);
-      const double tr = d.pos[a * cap + i] + dr;
-      d.delta[a * cap + i] += dr;
-      p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
-      d.pos[a * cap + i] = p[a];
-    } else {
-      p[a] = d.pos[a * cap + i];
+    for (int a = 0; a < 3; ++a) {
+      const double v = d.vel[a * cap + i];
+      const double ac = (double)d.acc[a * cap + i];
+      if (!predict_only) {
+        const double dr = dt * (v + 0.5 * dt * ac);
+        const double tr = d.pos[a * cap + i] + dr;
+        d.delta[a * cap + i] += dr;
+        p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
+        d.pos[a * cap + i] = p[a];
+      } else {
+        p[a] = d.pos[a * cap + i];
+      }
+      d.vbuf[a * cap + i] = v + 0.5 * dt * ac;
+      fin = fin && isfinite(p[a]) && isfinite(v);
     }
-    d.vbuf[a * cap + i] = v + 0.5 * dt * ac;
-    fin = fin && isfinite(p[a]) && isfinite(v);
+    if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
+    if (keys) {
+      k = cell_of(R, p);
+      d.key[i] = k;
+    }
   }
-  if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
-  if (keys) {
-    const uint32_t k = cell_of(R, p);
-    d.key[i] = k;
-    atomicAdd(&d.cell_cnt[k], 1u);
-  }
+  if (keys) (void)agg_inc(d.cell_cnt, k, active);
 }
 void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st) {
   if (!d.n) return;
@@ -95,15 +119,18 @@ void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t s
 }
 
 __global__ void __launch_bounds__(256) k_keys(DevState d) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
-  const Replica& R = d.reps[r];
-  double p[3];
-  for (int a = 0; a < 3; ++a) p[a] = d.pos[a * d.cap + i];
-  const uint32_t k = cell_of(R, p);
-  d.key[i] = k;
-  atomicAdd(&d.cell_cnt[k], 1u);
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i0 < d.n;
+  uint32_t k = 0;
+  if (active) {
+    const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i0) : 0;
+    const Replica& R = d.reps[r];
+    double p[3];
+    for (int a = 0; a < 3; ++a) p[a] = d.pos[a * d.cap + i0];
+    k = cell_of(R, p);
+    d.key[i0] = k;
+  }
+  (void)agg_inc(d.cell_cnt, k, active);
 }
 void launch_keys(const DevState& d, hipStream_t st) {
   if (!d.n) return;
@@ -207,9 +234,10 @@ void launch_scan_cells(const DevState& d, hipStream_t st) {
 
 __global__ void __launch_bounds__(256) k_scatter(DevState d) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  const uint32_t slot = atomicAdd(&d.cursor[d.key[i]], 1u);
-  d.perm[slot] = i;
+  const bool active = i < d.n;
+  const uint32_t k = active ? d.key[i] : 0u;
+  const uint32_t slot = agg_inc(d.cursor, k, active);
+  if (active) d.perm[slot] = i;
 }
 void launch_scatter(const DevState& d, hipStream_t st) {
   if (!d.n) return;
@@ -238,29 +266,64 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
   hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d);
 }
 
-// sorted FP32 copies: position in the bead's own cell, predicted velocity
-__global__ void __launch_bounds__(256) k_gather(DevState d, const double* __restrict__ vsrc) {
+// Moves every bead to its cell-order slot s (from old slot perm[s]) in the
+// other master buffer set, and writes the FP32 copies the pair passes read:
+// position inside the bead's cell (FP64 difference, then rounded) with
+// id << 2 | type in .w, and the velocity the force uses.
+__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const uint32_t i = d.perm[s];
+  d.inv_perm[i] = s;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const uint32_t key = d.key[i];
   const uint32_t c = key - R.cell_off;
   const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
-  const uint32_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+  const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
   const uint32_t cap = d.cap;
-  const float lx = (float)(d.pos[i] - (double)cx * R.h[0]);
-  const float ly = (float)(d.pos[cap + i] - (double)cy * R.h[1]);
-  const float lz = (float)(d.pos[2 * cap + i] - (double)cz * R.h[2]);
-  const uint32_t w = (d.id[i] << 2) | (d.prop[i] & 3u);
-  d.spos[s] = make_float4(lx, ly, lz, __uint_as_float(w));
-  d.svel[s] = make_float4((float)vsrc[i], (float)vsrc[cap + i], (float)vsrc[2 * cap + i], 0.f);
+  float l[3], v[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double x = d.pos[a * cap + i];
+    const double vb = d.vbuf[a * cap + i];
+    d.alt.pos[a * cap + s] = x;
+    d.alt.vbuf[a * cap + s] = vb;
+    d.alt.delta[a * cap + s] = d.delta[a * cap + i];
+    l[a] = (float)(x - (double)cc[a] * R.h[a]);
+    if (full) {
+      d.alt.vel[a * cap + s] = d.vel[a * cap + i];
+      d.alt.acc[a * cap + s] = d.acc[a * cap + i];
+      d.alt.press[a * cap + s] = d.press[a * cap + i];
+    }
+    v[a] = (float)(predicted ? vb : d.vel[a * cap + i]);
+  }
+  const uint32_t id = d.id[i], pr = d.prop[i];
+  d.alt.id[s] = id;
+  d.alt.prop[s] = pr;
+  d.alt.amp_id[s] = d.amp_id[i];
+  d.alt.unit[s] = d.unit[i];
+  d.alt.orig[s] = d.orig[i];
+  d.alt.chain[s] = d.chain[i];
+  d.spos[s] = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
+  d.svel[s] = make_float4(v[0], v[1], v[2], 0.f);
   d.scell[s] = key;
 }
-void launch_gather(const DevState& d, const double* vsrc, hipStream_t st) {
+void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_gather, grid1(d.n, 256), dim3(256), 0, st, d, vsrc);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
+                     full ? 1 : 0);
+}
+
+// chain bead slots follow the beads to their new slots
+__global__ void __launch_bounds__(256) k_topo_remap(DevState d) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 4u * d.n_amp) return;
+  d.topo[k] = d.inv_perm[d.topo[k]];
+}
+void launch_topo_remap(const DevState& d, hipStream_t st) {
+  if (!d.n_amp) return;
+  hipLaunchKernelGGL(k_topo_remap, grid1(4 * d.n_amp, 256), dim3(256), 0, st, d);
 }
 
 // ---------------------------------------------------------------- kick
@@ -383,8 +446,14 @@ void launch_cm_drift(const DevState& d, hipStream_t st) {
 //  reference's solvent loop, observer.hpp:171); 11 sum over amphiphiles of
 //  |mean delta of beads 4a..4a+3|^2 (observer.hpp:160-167); 12 number of
 //  pair-list entries (sorted slots of the replica).
+__global__ void __launch_bounds__(256) k_inv_orig(DevState d, uint32_t* __restrict__ inv_orig) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.n) inv_orig[d.orig[i]] = i;
+}
+
 __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
                                                          const uint32_t* __restrict__ amp_num,
+                                                         const uint32_t* __restrict__ inv_orig,
                                                          double* __restrict__ part) {
 #pragma clang fp contract(off)
   const int r = blockIdx.y;
@@ -409,14 +478,19 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
     }
     acc[9].add(dd);
     acc[12].add((double)min(d.ncount[i], (uint32_t)d.W));
-    if (k >= na) acc[10].add(dd);
+    // the reference indexes beads by their load order: solvent loop from
+    // index amp_num (observer.hpp:171), amphiphile a = beads 4a..4a+3 (:163-164)
+    if (d.orig[i] - b0 >= na) acc[10].add(dd);
     if ((k & 3u) == 0 && (k >> 2) < na && k + 3 < n_r) {
+      uint32_t q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = inv_orig[b0 + k + u];
       double m[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        m[a] = d.delta[a * cap + i] + d.delta[a * cap + i + 1];
-        m[a] += d.delta[a * cap + i + 2];
-        m[a] += d.delta[a * cap + i + 3];
+        m[a] = d.delta[a * cap + q[0]] + d.delta[a * cap + q[1]];
+        m[a] += d.delta[a * cap + q[2]];
+        m[a] += d.delta[a * cap + q[3]];
         m[a] /= 4.0;
       }
       acc[11].add(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
@@ -425,9 +499,11 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
   block_reduce_write<13>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 2 * 13);
 }
 
-void launch_observe(const DevState& d, const uint32_t* amp_num, double* out, hipStream_t st) {
+void launch_observe(const DevState& d, const uint32_t* amp_num, uint32_t* inv_orig, double* out,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_inv_orig, grid1(d.n, 256), dim3(256), 0, st, d, inv_orig);
   hipLaunchKernelGGL(k_observe, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d, amp_num,
-                     d.red);
+                     (const uint32_t*)inv_orig, d.red);
   hipLaunchKernelGGL(k_red_final<13>, dim3(d.n_rep), dim3(64), 0, st, d.red, out);
 }
 
@@ -442,22 +518,24 @@ __device__ __forceinline__ void bd_min_image(const Replica& R, double d[3]) {
 }
 
 
-__global__ void __launch_bounds__(256) k_bonded(DevState d, int virial, double* __restrict__ vir_chain) {
+// Bonded terms of chain a (ForceBonded::CalcBondBendGlobalCell<4>,
+// bdf_calculator.hpp:153-189, with MinImage :55-59, the bond :95-115 and
+// the bend :117-150), FP64 in the reference's order: F[u] receives its terms
+// in exactly the sequence of the reference loop.
+__device__ __forceinline__ void chain_terms(const DevState& d, const Replica& R, uint32_t a,
+                                            double F[4][3], double vir[3]) {
 #pragma clang fp contract(off)
-  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= d.n_amp) return;
-  const Replica& R = d.reps[d.amp_rep[a]];
   const uint32_t cap = d.cap;
   uint32_t l[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) l[u] = d.topo[4 * a + u];
   const double cf_b = d.cf_bend[a];
-  double F[4][3];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int c = 0; c < 3; ++c) F[u][c] = 0.0;
-  double dr[3][3], inv_dr[3], vir[3] = {0.0, 0.0, 0.0};
+  double dr[3][3], inv_dr[3];
+  vir[0] = vir[1] = vir[2] = 0.0;
 #pragma unroll
   for (int u = 1; u < 4; ++u) {
 #pragma unroll
@@ -466,7 +544,6 @@ __global__ void __launch_bounds__(256) k_bonded(DevState d, int virial, double* 
     const double dist2 = dr[u - 1][0] * dr[u - 1][0] + dr[u - 1][1] * dr[u - 1][1] +
                          dr[u - 1][2] * dr[u - 1][2];
     inv_dr[u - 1] = 1.0 / sqrt(dist2);
-    // bond, bdf_calculator.hpp:95-115
     const double cf_bond = R.cf_s * (inv_dr[u - 1] - R.ibond);
     double Fb[3];
 #pragma unroll
@@ -479,7 +556,6 @@ __global__ void __launch_bounds__(256) k_bonded(DevState d, int virial, double* 
       F[u][c] += Fb[c];
     }
     if (u >= 2) {
-      // bend over beads u-2, u-1, u, bdf_calculator.hpp:117-150
       const double* d0 = dr[u - 2];
       const double* d1 = dr[u - 1];
       const double inv_dr_prod = inv_dr[u - 2] * inv_dr[u - 1];
@@ -504,45 +580,74 @@ __global__ void __launch_bounds__(256) k_bonded(DevState d, int virial, double* 
       }
     }
   }
+}
+
+// One thread per bead: the bead evaluates its own chain (four beads, a few
+// hundred FP64 flops) and keeps its own row, so every write is to the bead's
+// own slot; then kick (driftkick.hpp:32-40) from the same registers.  The
+// chain virial is counted once, by the chain's first bead.
+__global__ void __launch_bounds__(256) k_bonded_kick(DevState d, int virial, int kick,
+                                                     double* __restrict__ vir_bead) {
+#pragma clang fp contract(off)
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const uint32_t cap = d.cap;
+  const uint32_t ch = d.chain[s];
+  double vir[3] = {0.0, 0.0, 0.0};
+  float af[3];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int c = 0; c < 3; ++c) af[c] = d.acc[c * cap + s];
+  if (ch != 0xffffffffu) {
+    const uint32_t u = ch & 3u;
+    double F[4][3];
+    chain_terms(d, R, ch >> 2, F, vir);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      float* p = &d.acc[c * cap + l[u]];
-      *p = (float)((double)*p + F[u][c]);
+      af[c] = (float)((double)af[c] + F[u][c]);
+      d.acc[c * cap + s] = af[c];
     }
-  if (virial) {
+    if (u != 0) vir[0] = vir[1] = vir[2] = 0.0;
+  }
+  if (virial)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) vir_chain[3 * (size_t)a + c] = vir[c];
+    for (int c = 0; c < 3; ++c) vir_bead[3 * (size_t)s + c] = vir[c];
+  if (kick) {
+    const double dt = R.dt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      d.vel[c * cap + s] = d.vbuf[c * cap + s] + dt * 0.5 * (double)af[c];
   }
 }
 
-// per-replica sum of the chain virials (chains of a replica are contiguous)
-__global__ void __launch_bounds__(kRedThreads) k_bonded_vir(const double* __restrict__ vir_chain,
-                                                            const uint32_t* __restrict__ amp_off,
-                                                            double* __restrict__ part) {
+// per-replica sum of the per-bead chain virials
+__global__ void __launch_bounds__(kRedThreads) k_bead_vir(DevState d, const double* __restrict__ vb,
+                                                          double* __restrict__ part) {
   const int r = blockIdx.y;
-  const uint32_t a0 = amp_off[r], na = amp_off[r + 1] - a0;
+  const uint32_t b0 = d.rep_off[r], nr = d.rep_off[r + 1] - b0;
   KSum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
-  for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < na; k += kRedBlocks * kRedThreads)
+  for (uint32_t k = blockIdx.x * kRedThreads + threadIdx.x; k < nr; k += kRedBlocks * kRedThreads)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[c].add(vir_chain[3 * (size_t)(a0 + k) + c]);
+    for (int c = 0; c < 3; ++c) acc[c].add(vb[3 * (size_t)(b0 + k) + c]);
   block_reduce_write<3>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 6);
 }
 
-void launch_bonded(const DevState& d, bool virial, double* vir_out, hipStream_t st) {
-  // vir_out: device [n_rep][3]; the chain scratch lives after the partials
-  if (d.n_amp) {
-    double* vir_chain = d.red + (size_t)d.n_rep * kRedBlocks * 2 * 16;
-    hipLaunchKernelGGL(k_bonded, grid1(d.n_amp, 256), dim3(256), 0, st, d, virial ? 1 : 0,
-                       vir_chain);
-    if (virial) {
-      hipLaunchKernelGGL(k_bonded_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st,
-                         vir_chain, d.amp_off, d.red);
+void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_out,
+                        hipStream_t st) {
+  if (!d.n) return;
+  double* vir_bead = d.red + (size_t)d.n_rep * kRedBlocks * 2 * 16;
+  if (d.n_amp || kick)
+    hipLaunchKernelGGL(k_bonded_kick, grid1(d.n, 256), dim3(256), 0, st, d, virial ? 1 : 0,
+                       kick ? 1 : 0, vir_bead);
+  if (virial) {
+    if (d.n_amp) {
+      hipLaunchKernelGGL(k_bead_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d,
+                         (const double*)vir_bead, d.red);
       hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out);
+    } else {
+      hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
     }
-  } else if (virial) {
-    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
   }
 }
 
@@ -632,7 +737,7 @@ __global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, ui
     const uint32_t b0 = d.rep_off[r], b1 = d.rep_off[r + 1];
     for (uint32_t j = b0; j < b1; ++j) visit(j);
   } else {
-    const uint32_t c = d.key[i] - R.cell_off;
+    const uint32_t c = d.scell[i] - R.cell_off;
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     const int cx = (int)(c % (uint32_t)nx), cy = (int)((c / (uint32_t)nx) % (uint32_t)ny),
               cz = (int)(c / (uint32_t)(nx * ny));
@@ -641,7 +746,7 @@ __global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, ui
         for (int ox = -rad[0]; ox <= rad[0]; ++ox) {
           const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
           const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-          for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) visit(d.perm[t]);
+          for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) visit(t);
         }
   }
   for (int a = 0; a < 3; ++a) {
@@ -769,10 +874,12 @@ __global__ void k_chem_insert(DevState d, const uint32_t* __restrict__ targets, 
     d.delta[a * cap + i] = 0.0;
   }
   d.id[i] = i;
+  d.orig[i] = i;
   d.prop[i] = (u == 0) ? 0u : 1u;
   d.amp_id[i] = new_amp;
   d.unit[i] = u;
   d.topo[4 * (size_t)new_amp + u] = i;
+  d.chain[i] = (new_amp << 2) | u;
   if (u == 0) d.amp_rep[new_amp] = 0;
 }
 void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget, double eps,

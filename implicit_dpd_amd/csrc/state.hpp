@@ -4,8 +4,11 @@
 
 namespace mdpd {
 
-struct DevState {
-  // master bead storage, slot order (upload order), SoA with stride cap
+// Per-bead master storage, SoA with stride cap.  Bead storage is kept in
+// cell order: every cell-list rebuild permutes it into the new sorted order
+// (a near-identity permutation, so the copies stream), and `orig` carries
+// each bead's upload slot for the host.
+struct Master {
   double* pos;      // [3][cap]
   double* vel;      // [3][cap]  velocity after the last kick
   double* vbuf;     // [3][cap]  predicted velocity = vel_buf (driftkick.hpp:25-27)
@@ -16,6 +19,12 @@ struct DevState {
   uint32_t* prop;
   uint32_t* amp_id;
   uint32_t* unit;
+  uint32_t* orig;   // upload slot of the bead in this storage slot
+  uint32_t* chain;  // chain index << 2 | position in the chain, or ~0u
+};
+
+struct DevState : Master {
+  Master alt;       // the other buffer set, target of the next permutation
   uint32_t cap;
   uint32_t n;       // beads in use
   // cell list, rebuilt every step
@@ -24,12 +33,13 @@ struct DevState {
   uint32_t* cell_start; // [ncell + 1]
   uint32_t* cursor;     // [ncell]
   uint32_t* bsum;       // scan block sums
-  uint32_t* perm;       // [cap] sorted slot -> bead slot
+  uint32_t* perm;       // [cap] new slot -> old slot of the last rebuild
+  uint32_t* inv_perm;   // [cap] old slot -> new slot
   uint32_t ncell;
-  // sorted FP32 copies
+  // FP32 copies in cell order (same slots as the master storage)
   float4* spos;         // local coordinates in the bead's cell, .w = id<<2 | type
   float4* svel;         // predicted velocity
-  uint32_t* scell;      // global cell of each sorted slot
+  uint32_t* scell;      // global cell of each slot at the last rebuild
   float* sdens;         // [nd][cap] density
   uint32_t* nlist;      // wave-tiled pair list [cap/64][W][64]
   uint32_t* ncount;     // [cap]
@@ -60,20 +70,24 @@ void launch_keys(const DevState& d, hipStream_t st);
 void launch_scan_cells(const DevState& d, hipStream_t st);
 void launch_scatter(const DevState& d, hipStream_t st);
 void launch_cell_sort(const DevState& d, hipStream_t st);
-void launch_gather(const DevState& d, const double* vsrc, hipStream_t st);
+// permutes the master storage into d.alt in cell order and writes the FP32
+// copies; `full` also carries vel, acc and press (needed when they are live)
+void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st);
+void launch_topo_remap(const DevState& d, hipStream_t st);
 void launch_density(const DevState& d, int ns, int nd, int block, hipStream_t st);
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
                   hipStream_t st);
-void launch_bonded(const DevState& d, bool virial, double* vir_out /*[n_rep][3] device*/,
-                   hipStream_t st);
 void launch_kick(const DevState& d, hipStream_t st);
+// bonded forces of every bead's own chain added to its acceleration, then
+// the kick, in one pass over the beads (chain virial on virial steps)
+void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_out,
+                        hipStream_t st);
 void launch_cm_drift(const DevState& d, hipStream_t st);
 void launch_head_conversion(const DevState& d, hipStream_t st);
 void launch_time_inc(uint32_t* d_time, hipStream_t st);
-// observables: out[n_rep][kObsQ] FP64
-constexpr int kObsQ = 16;
-void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, double* out,
-                    hipStream_t st);
+// observables: out[n_rep][13] FP64; inv_orig scratch [cap]
+void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, uint32_t* inv_orig,
+                    double* out, hipStream_t st);
 void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist,
                             hipStream_t st);
 void launch_neighbor_cm(const DevState& d, const uint32_t* beads, uint32_t nb, double* cm_out,
@@ -91,7 +105,6 @@ void launch_chem_flags(const DevState& d, const double* core /*[n_core][6]: pos 
                        hipStream_t st);
 void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget,
                         double eps, double bond_leng, uint32_t time, hipStream_t st);
-// snapshot of master state (autotuner)
 void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st);
 
 }  // namespace mdpd
