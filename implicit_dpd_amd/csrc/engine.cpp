@@ -707,7 +707,8 @@ int mdpd_set_tuning(mdpd_ctx* c, const mdpd_tuning* t) {
   auto okb = [](int b) { return b == 64 || b == 128 || b == 256; };
   if (!okb(t->block_scan) || !okb(t->block_force)) return fail(c, MDPD_ERR_ARG, "block sizes must be 64, 128 or 256");
   if (t->cell_div != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "cell_div must be 1");
-  if (t->nlist_width < 8 || t->nlist_width > 1024) return fail(c, MDPD_ERR_ARG, "nlist_width must be 8..1024");
+  if (t->nlist_width < 8 || t->nlist_width > 1024 || (t->nlist_width & 3))
+    return fail(c, MDPD_ERR_ARG, "nlist_width must be a multiple of 4 in 8..1024");
   if (t->nlist_width != c->tune.nlist_width) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     const uint32_t cap64 = div_up(c->cap, 64) * 64;
@@ -899,7 +900,9 @@ int mdpd_neighbor_pairs(mdpd_ctx* c, double cutoff, uint32_t* out_i, uint32_t* o
     for (uint32_t s = 0; s < n; ++s) {
       if (cnt[s] > W) return fail(c, MDPD_ERR_NLIST_OVERFLOW, "pair list overflow");
       for (uint32_t k = 0; k < cnt[s]; ++k) {
-        const uint32_t e = nl[(size_t)(s >> 6) * W * 64 + (size_t)k * 64 + (s & 63)];
+        // chunks of four entries: [tile][k / 4][lane][k % 4]
+        const uint32_t e = nl[((size_t)(s >> 6) * W * 64 + (size_t)(k >> 2) * 256 +
+                               (size_t)(s & 63) * 4) + (k & 3)];
         keys.push_back(((uint64_t)perm[s] << 32) | perm[e & kIndexMask]);
       }
     }

@@ -2,19 +2,20 @@
 // nbdf_calculator.hpp:11-61), which also records the pair list, and the
 // force pass (CalcForceEpEpDPD, nbdf_calculator.hpp:63-145).
 //
-// One thread per sorted bead (cell order), gathering over the 27 cells
-// around its own; no Newton-3 and no force atomics: every bead sums its own
-// pairs, as the reference does.  Positions are FP32 offsets inside the
-// bead's cell, so a pair displacement is (l_i - o*h) - l_j with o the
-// stencil offset, accurate to a few 1e-7; pairs whose r^2 lies within
-// kGuard of a cutoff are decided again in FP64 from the master positions
-// with the oracle's minimum-image expression, so the pair sets are those of
-// the FP64 oracle.
+// One thread per bead in cell order, gathering over the 27 cells around its
+// own; no Newton-3 and no force atomics: every bead sums its own pairs, as
+// the reference does.  Positions are FP32 offsets inside the bead's cell, so
+// a pair displacement is (l_i - o*h) - l_j with o the stencil offset,
+// accurate to a few 1e-7; pairs whose r^2 lies within kGuard of a cutoff are
+// decided again in FP64 from the master positions with the oracle's
+// minimum-image expression, so the pair sets are those of the FP64 oracle.
 //
-// The density pass writes, per 64-bead tile (one wavefront), a column-major
-// list of the in-range neighbours (sorted slot | stencil code << 26); the
-// force pass walks that list, so the Saru/log/cos/sqrt work of the force
-// runs only on pairs inside the cutoff.
+// The density pass writes, per 64-bead tile (one wavefront), the in-range
+// neighbours (slot | stencil code << 26) in chunks of four: chunk c of lane
+// l sits at [tile][c][l] (16 bytes), so every store is a whole 16-byte
+// chunk and the force pass reads a wave's chunk row as one contiguous 1 KiB
+// block.  The force pass walks that list, so the Saru/log/cos/sqrt work runs
+// only on pairs inside the cutoff.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -46,41 +47,49 @@ __device__ __forceinline__ bool decide(float r2, float cut2f, double cut2, const
   return fp64_r2(d, R, s, t) < cut2;
 }
 
+__device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s) {
+  return reinterpret_cast<const uint4*>(d.nlist) + (size_t)(s >> 6) * ((uint32_t)d.W >> 2) * 64u +
+         (s & 63u);
+}
+
 template <int ND, bool MULTI>
 __global__ void __launch_bounds__(256) k_density(DevState d) {
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
-  const float4 pi = d.spos[s];
-  const uint32_t c = d.scell[s] - R.cell_off;
+  // replica constants in registers for the loops below
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+  const uint32_t cell_off = R.cell_off;
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
+  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
+  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
+  const float4 pi = d.spos[s];
+  const uint32_t c = d.scell[s] - cell_off;
   const int cx = (int)(c % (uint32_t)nx);
   const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
   const int cz = (int)(c / (uint32_t)(nx * ny));
-  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
-  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
   const uint32_t W = (uint32_t)d.W;
-  uint32_t* lst = d.nlist + (size_t)(s >> 6) * W * 64u + (s & 63u);
+  uint4* lst = const_cast<uint4*>(tile_list(d, s));
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
-  uint32_t cnt = 0;
+  uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
+  const int npiece = (cx >= 1 && cx <= nx - 2) ? 1 : 2;
   for (int oz = -1; oz <= 1; ++oz) {
     int qz = cz + oz;
     qz += (qz < 0) ? nz : 0;
     qz -= (qz >= nz) ? nz : 0;
-    const float bz = pi.z - (float)oz * R.hf[2];
+    const float bz = pi.z - (float)oz * hz;
     for (int oy = -1; oy <= 1; ++oy) {
       int qy = cy + oy;
       qy += (qy < 0) ? ny : 0;
       qy -= (qy >= ny) ? ny : 0;
-      const float by = pi.y - (float)oy * R.hf[1];
+      const float by = pi.y - (float)oy * hy;
       // The three x-neighbour cells of a row are consecutive in cell order,
       // hence one contiguous run of slots (two runs where the row wraps).
-      const uint32_t rowq = R.cell_off + (uint32_t)((qz * ny + qy) * nx);
+      const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      const int npiece = (cx >= 1 && cx <= nx - 2) ? 1 : 2;
       for (int pc = 0; pc < npiece; ++pc) {
         int xa, xb, oa;
         if (npiece == 1) { xa = cx - 1; xb = cx + 1; oa = -1; }
@@ -90,36 +99,44 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
         const uint32_t b1 = (xb > xa) ? d.cell_start[rowq + xa + 1] : t1;
         const uint32_t b2 = (xb > xa + 1) ? d.cell_start[rowq + xa + 2] : t1;
         for (uint32_t t = t0; t < t1; ++t) {
-          if (t == s) continue;
           const int ox = oa + (t >= b1 ? 1 : 0) + (t >= b2 ? 1 : 0);
-          const float bx = pi.x - (float)ox * R.hf[0];
-          const uint32_t code = cyz | ((uint32_t)(ox + 1) << kCodeShift);
+          const float bx = pi.x - (float)ox * hx;
           const float4 pj = d.spos[t];
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 >= cut_hi) continue;
+          // t == s gives r2 == 0 and is dropped with the self pair below
+          if (r2 >= cut_hi || t == s) continue;
           const bool in_rc = decide(r2, 1.0f, 1.0, d, R, s, t);
           bool in_rd = decide(r2, rd2f, R.rd2, d, R, s, t);
           if (in_rd && r2 < 1e-6f) in_rd = fp64_r2(d, R, s, t) != 0.0;
           if (in_rd) {
             const float rr = __builtin_amdgcn_sqrtf(r2);
             const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
-            if (ND == 1) rho[0] += crho * w * w;
+            const float v = crho * w * w;
+            if (ND == 1) rho[0] += v;
             else {
               const uint32_t tj = __float_as_uint(pj.w) & 3u;
-              const float v = crho * w * w;
 #pragma unroll
               for (int k = 0; k < ND; ++k) rho[k] += (tj == (uint32_t)k) ? v : 0.f;
             }
           }
           if (in_rc) {
-            if (cnt < W) lst[(size_t)cnt * 64u] = t | code;
+            const uint32_t e = t | cyz | ((uint32_t)(ox + 1) << kCodeShift);
+            const uint32_t k = cnt & 3u;
+            if (k == 3u) {
+              if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
+            } else {
+              e0 = (k == 0u) ? e : e0;
+              e1 = (k == 1u) ? e : e1;
+              e2 = (k == 2u) ? e : e2;
+            }
             ++cnt;
           }
         }
       }
     }
   }
+  if ((cnt & 3u) != 0u && cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, 0u);
 #pragma unroll
   for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
   d.ncount[s] = cnt;
@@ -178,10 +195,9 @@ __global__ void __launch_bounds__(256) k_force(DevState d) {
   const uint32_t W = (uint32_t)d.W;
   uint32_t cnt = d.ncount[s];
   cnt = cnt < W ? cnt : W;
-  const uint32_t* lst = d.nlist + (size_t)(s >> 6) * W * 64u + (s & 63u);
+  const uint4* lst = tile_list(d, s);
   float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t e = lst[(size_t)k * 64u];
+  auto pair = [&](uint32_t e) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
     const float4 pj = d.spos[t];
@@ -222,15 +238,21 @@ __global__ void __launch_bounds__(256) k_force(DevState d) {
       py += fs * dy * dy;
       pz += fs * dz * dz;
     }
+  };
+  for (uint32_t k = 0; k < cnt; k += 4) {
+    const uint4 ch = lst[(size_t)(k >> 2) * 64u];
+    pair(ch.x);
+    if (k + 1 < cnt) pair(ch.y);
+    if (k + 2 < cnt) pair(ch.z);
+    if (k + 3 < cnt) pair(ch.w);
   }
-  const uint32_t i = s;
-  d.acc[i] = fx;
-  d.acc[d.cap + i] = fy;
-  d.acc[2 * d.cap + i] = fz;
+  d.acc[s] = fx;
+  d.acc[d.cap + s] = fy;
+  d.acc[2 * d.cap + s] = fz;
   if (VIRIAL) {
-    d.press[i] = px;
-    d.press[d.cap + i] = py;
-    d.press[2 * d.cap + i] = pz;
+    d.press[s] = px;
+    d.press[d.cap + s] = py;
+    d.press[2 * d.cap + s] = pz;
   }
   if (!isfinite(fx + fy + fz)) atomicOr(d.flags, FLAG_NONFINITE);
 }
@@ -263,8 +285,8 @@ void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int b
 }
 
 // ---------------------------------------------------------------- pair listing
-// Every ordered pair with r^2 < cutoff^2 as (bead slot i, bead slot j), for
-// the neighbour-set checks; same decision code as the density pass.
+// Every ordered pair with r^2 < cutoff^2 as (upload slot i, upload slot j),
+// for the neighbour-set checks; same decision code as the density pass.
 __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, double cut2,
                                                     uint32_t* __restrict__ oi,
                                                     uint32_t* __restrict__ oj,

@@ -65,42 +65,15 @@ __device__ __forceinline__ float neg_log_uni(uint32_t x) {
   return (m <= (1u << 23)) ? near1 : g;
 }
 
-// cos(2 pi v) for v = (k + 0.5) / 2^31 in (0, 1].  The argument is folded to
-// w in [0, 1/4] by symmetry (exactly, in FP32), then cos(2 pi w) or
-// sin(2 pi (1/4 - w)) by Taylor polynomials in z = (2 pi w)^2 / (pi/2 -
-// 2 pi w)^2 evaluated on the shorter side, |error| < 2e-7.
-__device__ __forceinline__ float cos_2pi_uni(uint32_t x) {
-  const uint32_t k = x >> 1;
-  // v = (k + 0.5) / 2^31; fold with integer arithmetic on 2k+1 over 2^32
-  const uint32_t q = 2u * k + 1u;            // v = q / 2^32, q odd
-  // w = distance of v to the nearest multiple of 1/2, sign of cos from
-  // which half-period: cos(2 pi v) = s * cos(2 pi w), w in [0, 1/4]
-  const uint32_t qh = q & 0x7fffffffu;        // v mod 1/2
-  const bool upper = qh > 0x40000000u;        // w = 1/2 - (v mod 1/2)
-  const uint32_t wq = upper ? (0x80000000u - qh) : qh;
-  // sign: cos negative for v in (1/4, 3/4)
-  const bool neg = (q > 0x40000000u) && (q < 0xC0000000u);
-  const float w = (float)wq * 2.3283064365386962890625e-10f;  // in [0, 1/4]
-  const float TWO_PI = 6.28318530717958647692f;
-  float r;
-  if (w <= 0.125f) {
-    const float a = TWO_PI * w;   // [0, pi/4]
-    const float z = a * a;
-    r = 1.0f + z * (-0.5f + z * (4.16666679e-2f + z * (-1.38888892e-3f + z * 2.48015876e-5f)));
-  } else {
-    const float a = TWO_PI * (0.25f - w);  // [0, pi/4]
-    const float z = a * a;
-    r = a * (1.0f + z * (-0.166666672f + z * (8.33333377e-3f + z * (-1.98412701e-4f + z * 2.75573188e-6f))));
-  }
-  return neg ? -r : r;
-}
-
-// FP32 Box-Muller normal of the pair noise: sqrt(-2 ln u1) cos(2 pi u2)
+// FP32 Box-Muller normal of the pair noise: sqrt(-2 ln u1) cos(2 pi u2).
+// The hardware cosine takes its argument in turns, so cos(2 pi u2) is one
+// v_cos_f32 of u2 in (0, 1] (absolute error below 3e-7 there).
 __device__ __forceinline__ float saru_nrml_f32(Saru& s) {
   const uint32_t x1 = saru_u32(s);
   const uint32_t x2 = saru_u32(s);
   const float r = __builtin_amdgcn_sqrtf(2.0f * neg_log_uni(x1));
-  return r * cos_2pi_uni(x2);
+  const float u2 = ((float)(int32_t)(x2 >> 1) + 0.5f) * 4.656612873077392578125e-10f;
+  return r * __builtin_amdgcn_cosf(u2);
 }
 
 }  // namespace mdpd

@@ -41,7 +41,7 @@ struct DevState : Master {
   float4* svel;         // predicted velocity
   uint32_t* scell;      // global cell of each slot at the last rebuild
   float* sdens;         // [nd][cap] density
-  uint32_t* nlist;      // wave-tiled pair list [cap/64][W][64]
+  uint32_t* nlist;      // wave-tiled pair list [cap/64][W/4][64][4]
   uint32_t* ncount;     // [cap]
   int32_t W;
   // replicas
