@@ -11,7 +11,9 @@ centre-of-mass drift removal every 100 steps, head conversion, clock.
 
 The timed region runs K steps with HIP events recorded on the engine stream
 around every phase; `value` is beads * K / (GPU time of the region).  The
-per-kernel averages of the same region give the force-pass roofline.  The CPU
+per-kernel averages of the same region give the roofline of the dominant pair
+pass (density or force, whichever takes longer); its HBM traffic and VALU
+counters come from the committed rocprofv3 summary of this command.  The CPU
 baseline is the FP64 oracle (the reference's per-bead gather loops, OpenMP)
 timed on a bounded sample of the same workload.
 """
@@ -27,11 +29,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MDPD bead-steps/s at 1M beads, % HBM roofline; surrogate vs plain calib runs"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # CUs x lanes/clk x clock (non-packed FP32/INT32)
+WORKLOAD = "membrane_1M"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def density_bytes_per_launch(n, n_pairs, nd, ncell):
+    """Algorithmic HBM bytes of one density pass: per bead its sorted
+    position (16) and cell (4) read, densities (4 nd) and pair count (4)
+    written; per pair-list entry 4 bytes written; the cell offsets (4 per
+    cell) read once.  Neighbour data reuse is on-chip and not counted."""
+    return n * (16 + 4 + 4 * nd + 4) + 4.0 * n_pairs + 4.0 * (ncell + 1)
 
 
 def force_bytes_per_launch(n, n_pairs, nd):
@@ -40,6 +50,32 @@ def force_bytes_per_launch(n, n_pairs, nd):
     map (4), the acceleration written (12); per pair-list entry 4 bytes.
     Neighbour data reuse is on-chip and not counted."""
     return n * (16 + 16 + 4 * nd + 4 + 4 + 12) + 4.0 * n_pairs
+
+
+def profiled_counters(workload):
+    """Per-launch HBM bytes and VALU counters of each kernel from the
+    committed rocprofv3 summary of this workload (profiles/), if present."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.txt")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or "=" not in line:
+                continue
+            kv = dict(tok.split("=", 1) for tok in line.split())
+            k = kv.get("variant", kv.get("kernel"))
+            out[k] = {"traffic_bytes": float(kv["traffic_bytes"]), "avg_ns": float(kv["avg_ns"]),
+                      "valu_busy": float(kv["valu_busy"]), "valu_util": float(kv["valu_util"])}
+    return out
+
+
+def pick(counters, prefix):
+    """The profiled variant of a kernel family with the most time (the
+    production variant, not the virial-step one)."""
+    c = [(v["avg_ns"], k, v) for k, v in counters.items()
+         if k.startswith(prefix) and not k.endswith(",true>")]
+    return max(c)[1:] if c else (None, None)
 
 
 def step_bytes_per_bead(nd, bonded, rho_mean):
@@ -88,7 +124,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU halo design is described in DESIGN.md)", file=sys.stderr)
+              "(spatial sharding with halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
 
     from implicit_dpd_amd import configs
@@ -139,22 +175,42 @@ def main():
     n_pairs = obs["n_pairs"]
     nd = p.n_dens
 
-    fb = force_bytes_per_launch(n, n_pairs, nd)
-    f_ms = ph["force"]
-    achieved = fb / (f_ms * 1e-3) / 1e9
+    ncell = 1
+    for a in range(3):
+        ncell *= max(int(math.floor(box[a])), 1)
+    prof = profiled_counters(WORKLOAD)
+    kern = {}
+    for name, phase, algo in (
+            ("k_density", "density", density_bytes_per_launch(n, n_pairs, nd, ncell)),
+            ("k_force", "force", force_bytes_per_launch(n, n_pairs, nd))):
+        ms = ph[phase]
+        gbs = algo / (ms * 1e-3) / 1e9
+        variant, pc = pick(prof, name)
+        kern[name] = {"ms": round(ms, 5), "algorithmic_bytes": int(algo), "achieved_gbs": round(gbs, 2),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "traffic": int(pc["traffic_bytes"]) if pc else None,
+                      "valu_busy_pct": pc["valu_busy"] if pc else None,
+                      "valu_lane_util_pct": pc["valu_util"] if pc else None,
+                      "profiled_variant": variant}
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    kd = kern[dom]
     bstep = step_bytes_per_bead(nd, True, rho_mean)
     roof = {
         "bound": "hbm",
-        "achieved": round(achieved, 2),
+        "achieved": kd["achieved_gbs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": "k_force (force pass)",
-        "kernel_ms": round(f_ms, 5),
-        "algorithmic_bytes_per_launch": int(fb),
+        "frac": kd["frac"],
+        "traffic": kd["traffic"],
+        "kernel": dom,
+        "kernel_ms": kd["ms"],
+        "algorithmic_bytes_per_launch": kd["algorithmic_bytes"],
         "pairs_per_bead": round(n_pairs / n, 3),
-        "binding_roof": "VALU (FP32/INT32): per pair Saru hash + log/sqrt/cos; see valu_* from rocprof",
+        "binding_roof": "VALU (FP32/INT32 lane ops: candidate tests, Saru hash, log/sqrt/cos); "
+                        "valu_busy_pct / valu_lane_util_pct from the committed rocprofv3 counters",
+        "valu_busy_pct": kd["valu_busy_pct"],
+        "valu_lane_util_pct": kd["valu_lane_util_pct"],
+        "pair_kernels": kern,
     }
     step_hbm = {"bytes_per_bead_step": round(bstep, 1),
                 "achieved_gbs": round(value * bstep / 1e9, 1),

@@ -54,7 +54,29 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_driver(verbose)
     return OUT
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "mdpd_main.cpp")
+DRIVER = os.path.join(HERE, "bin", "mdpd_main")
+
+
+def build_driver(verbose=True):
+    """The stand-alone host driver (C++ facade over the C ABI), linked
+    against the in-tree library with an rpath to it."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           DRIVER_SRC, "-o", DRIVER + ".tmp", "-L", HERE, "-lmdpd",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
 
 
 def clean():

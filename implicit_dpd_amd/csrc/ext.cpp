@@ -187,7 +187,7 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
   uint32_t* snapu = nullptr;
   HIPCHK(c, hipMalloc(&snap, 4 * n3 * sizeof(double)));
   HIPCHK(c, hipMalloc(&snapf, 2 * n3 * sizeof(float)));
-  HIPCHK(c, hipMalloc(&snapu, (5 * (size_t)d.cap + 4 * (size_t)c->amp_cap) * sizeof(uint32_t)));
+  HIPCHK(c, hipMalloc(&snapu, (6 * (size_t)d.cap + 4 * (size_t)c->amp_cap) * sizeof(uint32_t)));
   const uint32_t t0 = c->time;
   auto save = [&]() -> int {
     HIPCHK(c, hipMemcpyAsync(snap, d.pos, n3 * 8, hipMemcpyDeviceToDevice, c->st));
@@ -197,10 +197,11 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
     HIPCHK(c, hipMemcpyAsync(snapf, d.acc, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(snapf + n3, d.press, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     {
-      uint32_t* const src[5] = {d.prop, d.id, d.amp_id, d.unit, d.orig};
-      for (int q = 0; q < 5; ++q)
+      // every per-slot array the storage permutation moves, chain included
+      uint32_t* const src[6] = {d.prop, d.id, d.amp_id, d.unit, d.orig, d.chain};
+      for (int q = 0; q < 6; ++q)
         HIPCHK(c, hipMemcpyAsync(snapu + (size_t)q * d.cap, src[q], (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
-      HIPCHK(c, hipMemcpyAsync(snapu + 5 * (size_t)d.cap, d.topo, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(snapu + 6 * (size_t)d.cap, d.topo, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
     }
     return MDPD_OK;
   };
@@ -212,11 +213,14 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
     HIPCHK(c, hipMemcpyAsync(d.acc, snapf, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(d.press, snapf + n3, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     {
-      uint32_t* const dst[5] = {d.prop, d.id, d.amp_id, d.unit, d.orig};
-      for (int q = 0; q < 5; ++q)
+      uint32_t* const dst[6] = {d.prop, d.id, d.amp_id, d.unit, d.orig, d.chain};
+      for (int q = 0; q < 6; ++q)
         HIPCHK(c, hipMemcpyAsync(dst[q], snapu + (size_t)q * d.cap, (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
-      HIPCHK(c, hipMemcpyAsync(d.topo, snapu + 5 * (size_t)d.cap, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(d.topo, snapu + 6 * (size_t)d.cap, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
     }
+    // the cell list and pair list of the trial steps are stale now
+    c->cells_fresh = false;
+    c->have_density = false;
     c->time = t0;
     HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

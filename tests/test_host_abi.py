@@ -95,3 +95,14 @@ def test_xyz_roundtrip(tmp_path):
         assert np.allclose(r2[k], rec[k], rtol=1e-14, atol=0)
     for k in ("id", "prop", "amp_id", "unit"):
         assert np.array_equal(r2[k], rec[k])
+
+
+def test_driver_binary_links_and_checks_arguments():
+    """The stand-alone driver (C++ facade over the ABI) is built in-tree and
+    refuses a missing target directory like main.cpp:78-82."""
+    import subprocess
+    drv = os.path.join(ROOT, "implicit_dpd_amd", "bin", "mdpd_main")
+    assert os.path.exists(drv)
+    r = subprocess.run([drv], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "argv[1] = target directory." in r.stderr
