@@ -174,6 +174,17 @@ PROTOTYPES = {
     "mdpd_autotune": (C.c_int, [VP, C.c_uint32, P(MdpdTuning)]),
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
+    # equation-of-state surrogate
+    "mdpd_mlp_num_params": (C.c_int, [_U32P]),
+    "mdpd_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, P(VP)]),
+    "mdpd_mlp_destroy": (C.c_int, [VP]),
+    "mdpd_mlp_last_error": (C.c_char_p, [VP]),
+    "mdpd_mlp_set_params": (C.c_int, [VP, _FP, C.c_uint32]),
+    "mdpd_mlp_get_params": (C.c_int, [VP, _FP, C.c_uint32]),
+    "mdpd_mlp_forward": (C.c_int, [VP, _FP, C.c_uint32, _FP]),
+    "mdpd_mlp_loss_grad": (C.c_int, [VP, _FP, _FP, C.c_uint32, _DP, _FP]),
+    "mdpd_mlp_train": (C.c_int, [VP, _FP, _FP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                 C.c_uint32, _FP]),
 }
 
 _lib = None

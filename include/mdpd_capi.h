@@ -250,6 +250,35 @@ typedef struct mdpd_run_input {
 } mdpd_run_input;
 int mdpd_read_run_param(const char* dir, mdpd_params* p, mdpd_run_input* in);
 
+/* ---- equation-of-state surrogate (no reference counterpart) ------------
+   A 4 -> 128 -> 128 -> 2 perceptron with SiLU hidden layers that maps
+   (A, B, rd, kBT) (normalised by the caller) to (rho, P) of a plain run.
+   Products on the matrix cores with BF16 operands and FP32 accumulation;
+   FP32 master parameters, Adam.  Parameter layout (17,410 floats, weights
+   row-major [out][in]): W1[128][4], b1[128], W2[128][128], b2[128],
+   W3[2][128], b3[2].  Loss = sum over samples and outputs of
+   (y - t)^2 / (2 n). */
+typedef struct mdpd_mlp mdpd_mlp;
+int mdpd_mlp_num_params(uint32_t* n);
+/* max_batch: samples per device batch; parameters drawn from a seeded
+   U(-1/sqrt(fan_in), 1/sqrt(fan_in)) */
+int mdpd_mlp_create(uint32_t max_batch, uint32_t seed, mdpd_mlp** out);
+int mdpd_mlp_destroy(mdpd_mlp* m);
+const char* mdpd_mlp_last_error(const mdpd_mlp* m);
+int mdpd_mlp_set_params(mdpd_mlp* m, const float* params, uint32_t n);
+int mdpd_mlp_get_params(mdpd_mlp* m, float* params, uint32_t n);
+/* y[n][2] = f(x[n][4]), host arrays, any n */
+int mdpd_mlp_forward(mdpd_mlp* m, const float* x, uint32_t n, float* y);
+/* loss and its gradient w.r.t. the parameters over n <= max_batch samples;
+   grad may be NULL */
+int mdpd_mlp_loss_grad(mdpd_mlp* m, const float* x, const float* t, uint32_t n,
+                       double* loss, float* grad);
+/* Adam (beta 0.9/0.999, eps 1e-8) over mini-batches of `batch` samples of a
+   per-epoch shuffle (seeded); loss_per_epoch[epochs] may be NULL */
+int mdpd_mlp_train(mdpd_mlp* m, const float* x, const float* t, uint32_t n,
+                   uint32_t epochs, uint32_t batch, float lr, uint32_t seed,
+                   float* loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif
