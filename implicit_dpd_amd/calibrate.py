@@ -1,0 +1,225 @@
+"""Parameter calibration with the equation-of-state surrogate vs a plain sweep.
+
+Problem: Warren MDPD (F_C = A (1-r) + B (rho_i + rho_j)(1 - r/rd)) with
+B = 25, rd = 0.75, kBT = 1; find A such that a liquid slab coexisting with
+its vapour has bulk density rho = 6.0 (within 1%) and normal pressure
+P_zz = 0 (the coexistence condition).
+
+- A *plain run* is one slab system of ~200k beads (box L x L x 2L, liquid
+  started in the middle half at density 6), run on the GPU; it measures the
+  slab's bulk density from the z density profile and P_zz, averaged over the
+  second half of the run.
+- The *sweep* finds A by regula falsi on plain runs alone.
+- The *surrogate* is trained on ensemble runs (64 replicas of U[4k, 64k]
+  beads per launch with A ~ U[-60,-20], B ~ U[10,40], rd ~ U[0.65,0.85],
+  kBT = 1), proposes A by scanning the network, and each proposal is
+  verified by a plain run; a miss is added to the training set and the
+  network fine-tuned.
+
+The headline of `main()` is the number of plain runs each method needs.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import configs
+from .engine import Engine
+from .surrogate import Scaler, Surrogate
+
+TARGET_RHO = 6.0
+TOL = 0.01
+A_RANGE = (-60.0, -20.0)
+B_RANGE = (10.0, 40.0)
+RD_RANGE = (0.65, 0.85)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def slab_rho(profile, bin_vol):
+    """Bulk density of a slab from a z bead-count profile: bins above half
+    the peak form the slab (periodic); the two outermost bins on each side
+    are interface and dropped."""
+    rho = np.asarray(profile, np.float64) / bin_vol
+    inside = rho > 0.5 * rho.max()
+    if inside.all():
+        return float(rho.mean())
+    # rotate so that the slab does not wrap
+    start = int(np.argmin(inside))
+    rho, inside = np.roll(rho, -start), np.roll(inside, -start)
+    idx = np.nonzero(inside)[0]
+    core = idx[2:-2] if len(idx) > 6 else idx
+    return float(rho[core].mean()) if len(core) else float(rho.max())
+
+
+def measure(e, boxes, steps, every, nbins):
+    """Runs `steps` steps; over the second half, every `every` steps, sums
+    the z bead-count profile and P_zz of every replica.  Returns per replica
+    (rho_bulk, P_zz)."""
+    half = steps // 2
+    e.run(half)
+    nrep = len(boxes)
+    prof = np.zeros((nrep, nbins))
+    pzz = np.zeros(nrep)
+    k = 0
+    done = half
+    while done < steps:
+        m = min(every, steps - done)
+        e.run(m)
+        done += m
+        prof += e.density_profile(2, nbins)
+        pzz += np.array([o["pressure"][2] for o in e.observe_replicas()])
+        k += 1
+    out = []
+    for r, b in enumerate(boxes):
+        bin_vol = b[0] * b[1] * b[2] / nbins
+        out.append((slab_rho(prof[r] / k, bin_vol), pzz[r] / k))
+    return out
+
+
+def slab(A, B, rd, n, seed, rho0=6.0):
+    lx = max(4.0, math.ceil((n / rho0) ** (1.0 / 3.0)))
+    box = (lx, lx, 2.0 * lx)
+    n = int(rho0 * lx * lx * lx)
+    p = configs.warren_params(box, A=A, B=B, rd=rd, dt=0.005, pair_seed=seed)
+    s = configs.random_fluid(n, box, seed, zlim=(box[2] / 4, 3 * box[2] / 4))
+    return p, s, box
+
+
+def plain_run(A, B=25.0, rd=0.75, n=200000, steps=3000, every=50, seed=11):
+    p, s, box = slab(A, B, rd, n, seed)
+    e = Engine(p, capacity=len(s["pos"]))
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"])
+    e.set_clock(0, p.pair_seed)
+    (rho, pzz), = measure(e, [box], steps, every, int(box[2] / 0.5))
+    e.close()
+    return rho, pzz
+
+
+def ensemble_samples(n_launch, n_rep=64, nmin=4096, nmax=65536, steps=3000, every=100, seed=100):
+    X, Y = [], []
+    for b in range(n_launch):
+        rng = np.random.default_rng(seed + b)
+        params, systems, boxes, xs = [], [], [], []
+        for r in range(n_rep):
+            A = rng.uniform(*A_RANGE)
+            B = rng.uniform(*B_RANGE)
+            rd = rng.uniform(*RD_RANGE)
+            n = int(rng.integers(nmin, nmax + 1))
+            p, s, box = slab(A, B, rd, n, seed=(seed + b) * 1000 + r + 1)
+            params.append(p)
+            systems.append(s)
+            boxes.append(box)
+            xs.append((A, B, rd, 1.0))
+        allb = configs.concat_systems(systems)
+        e = Engine(params, n_beads=[len(s["pos"]) for s in systems])
+        e.upload(allb["pos"], allb["vel"], allb["ids"], allb["prop"])
+        e.set_clock(0, 0)
+        t0 = time.perf_counter()
+        res = measure(e, boxes, steps, every, 64)
+        e.close()
+        X += xs
+        Y += res
+        log(f"[calib] ensemble launch {b + 1}/{n_launch}: {sum(len(s['pos']) for s in systems)} "
+            f"beads, {time.perf_counter() - t0:.1f} s")
+    return np.array(X), np.array(Y)
+
+
+def sweep(plain, target=TARGET_RHO, lo=A_RANGE[0], hi=A_RANGE[1], max_runs=12):
+    """Regula falsi on A with plain runs; rho decreases as A rises."""
+    runs = []
+    f_lo = plain(lo)[0] - target
+    runs.append((lo, f_lo + target))
+    f_hi = plain(hi)[0] - target
+    runs.append((hi, f_hi + target))
+    a = None
+    while len(runs) < max_runs:
+        a = hi - f_hi * (hi - lo) / (f_hi - f_lo)
+        rho, _ = plain(a)
+        runs.append((a, rho))
+        f = rho - target
+        if abs(f) <= TOL * target:
+            return a, runs
+        if (f > 0) == (f_lo > 0):
+            lo, f_lo = a, f
+            f_hi *= 0.5  # Illinois step keeps both ends moving
+        else:
+            hi, f_hi = a, f
+            f_lo *= 0.5
+    return a, runs
+
+
+def surrogate_calibrate(X, Y, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6, seed=3,
+                        epochs=300):
+    xs, ys = Scaler(X), Scaler(Y)
+    net = Surrogate(max_batch=4096, seed=seed)
+    hist = net.train(xs.fwd(X), ys.fwd(Y), epochs=epochs, batch=256, lr=2e-3, seed=seed)
+    grid = np.linspace(A_RANGE[0], A_RANGE[1], 4001)
+    cand = np.stack([grid, np.full_like(grid, B), np.full_like(grid, rd), np.ones_like(grid)], 1)
+    runs = []
+    Xa, Ya = X.copy(), Y.copy()
+    a = None
+    for _ in range(max_runs):
+        pred = ys.inv(net.forward(xs.fwd(cand)))
+        a = float(grid[np.argmin(np.abs(pred[:, 0] - target))])
+        rho, pzz = plain(a)
+        runs.append((a, rho, float(pred[np.argmin(np.abs(grid - a)), 0])))
+        if abs(rho - target) <= TOL * target:
+            break
+        # add the verified point (weighted) and fine-tune
+        Xa = np.concatenate([Xa, np.repeat([[a, B, rd, 1.0]], 64, 0)])
+        Ya = np.concatenate([Ya, np.repeat([[rho, pzz]], 64, 0)])
+        net.train(xs.fwd(Xa), ys.fwd(Ya), epochs=40, batch=256, lr=1e-3, seed=seed + len(runs))
+    net.close()
+    return a, runs, hist
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="surrogate vs plain-sweep calibration")
+    ap.add_argument("--launches", type=int, default=64, help="ensemble launches of 64 replicas")
+    ap.add_argument("--nmin", type=int, default=4096)
+    ap.add_argument("--nmax", type=int, default=65536)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--plain-beads", type=int, default=200000)
+    ap.add_argument("--epochs", type=int, default=300)
+    args = ap.parse_args(argv)
+
+    gpu_s = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0}
+
+    def plain(a, key):
+        t0 = time.perf_counter()
+        r = plain_run(a, n=args.plain_beads, steps=args.steps)
+        gpu_s[key] += time.perf_counter() - t0
+        log(f"[calib] plain run A={a:.4f}: rho={r[0]:.4f} P_zz={r[1]:.4f}")
+        return r
+
+    t0 = time.perf_counter()
+    X, Y = ensemble_samples(args.launches, nmin=args.nmin, nmax=args.nmax, steps=args.steps)
+    gpu_s["samples"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    a_s, runs_s, hist = surrogate_calibrate(X, Y, lambda a: plain(a, "surrogate_runs"),
+                                            epochs=args.epochs)
+    gpu_s["train"] = time.perf_counter() - t0 - gpu_s["surrogate_runs"]
+    a_p, runs_p = sweep(lambda a: plain(a, "sweep_runs"))
+    out = {
+        "target": {"rho": TARGET_RHO, "tol": TOL, "B": 25.0, "rd": 0.75, "kBT": 1.0},
+        "samples": int(len(X)),
+        "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+        "surrogate": {"A": a_s, "plain_runs": len(runs_s),
+                      "runs": [{"A": a, "rho": r, "rho_pred": q} for a, r, q in runs_s]},
+        "sweep": {"A": a_p, "plain_runs": len(runs_p),
+                  "runs": [{"A": a, "rho": r} for a, r in runs_p]},
+        "plain_run_ratio": len(runs_p) / max(len(runs_s), 1),
+        "seconds": {k: round(v, 2) for k, v in gpu_s.items()},
+    }
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

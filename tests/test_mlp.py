@@ -149,3 +149,24 @@ def test_device_training_converges_and_is_reproducible():
     assert np.array_equal(p1, p2) and np.array_equal(h1, h2)
     assert L1 < 0.02 * L0, (L0, L1)
     assert h1[-1] < h1[0]
+
+
+# ---------------------------------------------------------------- calibration
+def test_slab_density_estimator():
+    from implicit_dpd_amd.calibrate import slab_rho
+    prof = np.zeros(40)
+    prof[30:40] = 6.0
+    prof[0:6] = 6.0          # slab wrapping through z = 0
+    prof[29] = prof[6] = 3.0  # interface bins
+    assert abs(slab_rho(prof, 1.0) - 6.0) < 1e-12
+    prof2 = prof * 2.0
+    assert abs(slab_rho(prof2, 2.0) - 6.0) < 1e-12
+
+
+@pytest.mark.gpu
+def test_calibration_pipeline_small():
+    from implicit_dpd_amd import calibrate as K
+    X, Y = K.ensemble_samples(1, n_rep=4, nmin=2000, nmax=3000, steps=200, every=50)
+    assert X.shape == (4, 4) and Y.shape == (4, 2) and np.all(np.isfinite(Y))
+    rho, pzz = K.plain_run(-40.0, n=20000, steps=400)
+    assert 3.0 < rho < 9.0 and np.isfinite(pzz)
