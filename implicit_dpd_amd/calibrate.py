@@ -118,6 +118,9 @@ def ensemble_samples(n_launch, n_rep=64, nmin=4096, nmax=65536, steps=3000, ever
             xs.append((A, B, rd, 1.0))
         allb = configs.concat_systems(systems)
         e = Engine(params, n_beads=[len(s["pos"]) for s in systems])
+        # strongly attractive, weakly repulsive corners of the sampled range
+        # condense to rho > 20: room for up to 256 neighbours per bead
+        e.set_tuning(nlist_width=256)
         e.upload(allb["pos"], allb["vel"], allb["ids"], allb["prop"])
         e.set_clock(0, 0)
         t0 = time.perf_counter()

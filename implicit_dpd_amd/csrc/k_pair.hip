@@ -74,8 +74,14 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
+  // The bead's x offset term for the left, own and right neighbour cell;
+  // a candidate picks one by comparing its slot with two run boundaries.
+  const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
+  // Pair-list entries are buffered in a three-deep shift register and
+  // written as whole 16-byte chunks of four.
   uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
-  const int npiece = (cx >= 1 && cx <= nx - 2) ? 1 : 2;
+  const bool wrap = !(cx >= 1 && cx <= nx - 2);
+  const uint32_t kInf = 0xFFFFFFFFu;
   for (int oz = -1; oz <= 1; ++oz) {
     int qz = cz + oz;
     qz += (qz < 0) ? nz : 0;
@@ -89,26 +95,38 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       // The three x-neighbour cells of a row are consecutive in cell order,
       // hence one contiguous run of slots (two runs where the row wraps).
       const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
+      const uint32_t* S = d.cell_start + rowq;
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      for (int pc = 0; pc < npiece; ++pc) {
-        int xa, xb, oa;
-        if (npiece == 1) { xa = cx - 1; xb = cx + 1; oa = -1; }
-        else if (cx == 0) { xa = pc ? 0 : nx - 1; xb = pc ? 1 : nx - 1; oa = pc ? 0 : -1; }
-        else { xa = pc ? 0 : nx - 2; xb = pc ? 0 : nx - 1; oa = pc ? 1 : -1; }
-        const uint32_t t0 = d.cell_start[rowq + xa], t1 = d.cell_start[rowq + xb + 1];
-        const uint32_t b1 = (xb > xa) ? d.cell_start[rowq + xa + 1] : t1;
-        const uint32_t b2 = (xb > xa + 1) ? d.cell_start[rowq + xa + 2] : t1;
-        for (uint32_t t = t0; t < t1; ++t) {
-          const int ox = oa + (t >= b1 ? 1 : 0) + (t >= b2 ? 1 : 0);
-          const float bx = pi.x - (float)ox * hx;
-          const float4 pj = d.spos[t];
+      const uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
+      for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
+        // run [ta, tb); slots >= b2 are in the right cell, >= b1 the own cell
+        uint32_t ta, tb, b1, b2;
+        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
+        else if (cx == 0) {
+          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; b1 = kInf; b2 = kInf; }
+          else { ta = S[0]; tb = S[2]; b1 = 0; b2 = S[1]; }
+        } else {
+          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
+          else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
+        }
+        float4 pn = ta < tb ? d.spos[ta] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t t = ta; t < tb; ++t) {
+          const float4 pj = pn;
+          if (t + 1 < tb) pn = d.spos[t + 1];  // next candidate in flight
+          const bool g2 = t >= b2, g1 = t >= b1;
+          const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
-          // t == s gives r2 == 0 and is dropped with the self pair below
-          if (r2 >= cut_hi || t == s) continue;
-          const bool in_rc = decide(r2, 1.0f, 1.0, d, R, s, t);
-          bool in_rd = decide(r2, rd2f, R.rd2, d, R, s, t);
-          if (in_rd && r2 < 1e-6f) in_rd = fp64_r2(d, R, s, t) != 0.0;
+          if (r2 >= cut_hi) continue;
+          bool in_rc = r2 < 1.0f, in_rd = r2 < rd2f;
+          // next to a cutoff, or (near-)coincident: decide in FP64; the
+          // bead itself (t == s, r2 == 0) lands here and is dropped
+          if (fabsf(r2 - 1.0f) <= kGuard || fabsf(r2 - rd2f) <= kGuard || r2 < 1e-6f) {
+            if (t == s) continue;
+            const double r2d = fp64_r2(d, R, s, t);
+            in_rc = r2d < 1.0;
+            in_rd = r2d < R.rd2 && r2d != 0.0;
+          }
           if (in_rd) {
             const float rr = __builtin_amdgcn_sqrtf(r2);
             const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
@@ -121,14 +139,13 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
             }
           }
           if (in_rc) {
-            const uint32_t e = t | cyz | ((uint32_t)(ox + 1) << kCodeShift);
-            const uint32_t k = cnt & 3u;
-            if (k == 3u) {
+            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
+            if ((cnt & 3u) == 3u) {
               if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
             } else {
-              e0 = (k == 0u) ? e : e0;
-              e1 = (k == 1u) ? e : e1;
-              e2 = (k == 2u) ? e : e2;
+              e0 = e1;
+              e1 = e2;
+              e2 = e;
             }
             ++cnt;
           }
@@ -136,7 +153,14 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       }
     }
   }
-  if ((cnt & 3u) != 0u && cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, 0u);
+  // the last (cnt % 4) entries sit at the right end of the shift register
+  const uint32_t tail = cnt & 3u;
+  if (tail != 0u && cnt < W) {
+    const uint4 ch = tail == 1u ? make_uint4(e2, 0u, 0u, 0u)
+                   : tail == 2u ? make_uint4(e1, e2, 0u, 0u)
+                                : make_uint4(e0, e1, e2, 0u);
+    lst[(size_t)(cnt >> 2) * 64u] = ch;
+  }
 #pragma unroll
   for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
   d.ncount[s] = cnt;
