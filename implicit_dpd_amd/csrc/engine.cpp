@@ -237,6 +237,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.rep_off, n_rep + 1);
   ALLOC(d.rep_cell_off, n_rep + 1);
   ALLOC(d.topo, 4 * (size_t)c->amp_cap);
+  ALLOC(d.chainpos, 4 * (size_t)c->amp_cap);
   ALLOC(d.cf_bend, c->amp_cap);
   ALLOC(d.amp_rep, c->amp_cap);
   ALLOC(d.amp_off, n_rep + 1);
@@ -289,7 +290,7 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
     if (prof) hipEventRecord(ev[k], st);
   };
   mark(0);
-  launch_fill_u32(d.cell_cnt, d.ncell + 1, 0u, st);
+  // the cell histogram is zero here: allocated zeroed and cleared by every scan
   launch_drift(d, true, false, st);
   mark(1);
   launch_scan_cells(d, st);
@@ -299,14 +300,14 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
   // bead storage moves to the new cell order; the slots below are the new ones
   launch_permute(d, true, false, st);
   swap_master(c);
-  launch_topo_remap(d, st);
   mark(3);
   launch_density(d, c->ns, c->nd, c->tune.block_scan, st);
   mark(4);
-  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st);
+  // the force pass ends with ForceBonded::CalcListedForce and the kick of
+  // each bead (bead-centric: every bead evaluates its own chain)
+  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true);
   mark(5);
-  // ForceBonded::CalcListedForce and kick fused (bead-centric, see k_step.hip)
-  launch_bonded_kick(d, virial, true, c->vir_bonded, st);
+  if (virial) launch_bead_vir_reduce(d, c->vir_bonded, st);
   mark(6);
   mark(7);
   if (cm) launch_cm_drift(d, st);
@@ -778,6 +779,8 @@ int mdpd_force(mdpd_ctx* c) {
 
 int mdpd_bonded(mdpd_ctx* c, double vir[3]) {
   if (!c) return MDPD_ERR_ARG;
+  // positions may have changed since the last permutation (drift, upload)
+  launch_chainpos_fill(c->d, c->st);
   launch_bonded_kick(c->d, true, false, c->vir_bonded, c->st);
   if (vir) {
     HIPCHK(c, hipMemcpyAsync(vir, c->vir_bonded, 24, hipMemcpyDeviceToHost, c->st));

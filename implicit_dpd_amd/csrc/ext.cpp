@@ -42,6 +42,8 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
   const uint32_t nc = cp->n_core;
   const double* box = c->R[0].box;
   std::vector<uint32_t> targets;
+  // the step keeps chain[] current, not topo: rebuild it for the slots now
+  launch_topo_from_chain(d, c->st);
   if (nc && d.n_amp) {
     uint32_t *ids = nullptr, *idx = nullptr;
     double *cm = nullptr, *core = nullptr;

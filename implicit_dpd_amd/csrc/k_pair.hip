@@ -15,13 +15,16 @@
 // l sits at [tile][c][l] (16 bytes), so every store is a whole 16-byte
 // chunk and the force pass reads a wave's chunk row as one contiguous 1 KiB
 // block.  The force pass walks that list, so the Saru/log/cos/sqrt work runs
-// only on pairs inside the cutoff.
+// only on pairs inside the cutoff; in the step it ends with the bead's own
+// bonded chain terms and the kick, so the acceleration never makes a round
+// trip through memory between the two.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "common.hpp"
 #include "saru.hpp"
 #include "state.hpp"
+#include "bonded.hpp"
 
 namespace mdpd {
 
@@ -190,7 +193,7 @@ __device__ __forceinline__ float sel(const float (&v)[NS], uint32_t t) {
 }
 
 template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
-__global__ void __launch_bounds__(256) k_force(DevState d) {
+__global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead) {
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -270,42 +273,51 @@ __global__ void __launch_bounds__(256) k_force(DevState d) {
     if (k + 2 < cnt) pair(ch.z);
     if (k + 3 < cnt) pair(ch.w);
   }
-  d.acc[s] = fx;
-  d.acc[d.cap + s] = fy;
-  d.acc[2 * d.cap + s] = fz;
   if (VIRIAL) {
     d.press[s] = px;
     d.press[d.cap + s] = py;
     d.press[2 * d.cap + s] = pz;
   }
   if (!isfinite(fx + fy + fz)) atomicOr(d.flags, FLAG_NONFINITE);
+  if (vir_bead) {
+    // ForceBonded::CalcListedForce and kick for this bead, from the FP32
+    // acceleration just summed (the same value the stand-alone pass reads)
+    float af[3] = {fx, fy, fz};
+    bonded_kick_bead(d, R, s, af, VIRIAL, true, vir_bead);
+  } else {
+    d.acc[s] = fx;
+    d.acc[d.cap + s] = fy;
+    d.acc[2 * d.cap + s] = fz;
+  }
 }
 
 template <int NS, int ND, int ENV>
-static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st) {
+static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb) {
   const dim3 g(div_up(d.n, block)), b(block);
   const bool multi = d.n_rep > 1;
   if (multi) {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb);
   } else {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb);
   }
 }
 
 template <int NS, int ND>
-static void force_launch2(const DevState& d, int env, bool virial, int block, hipStream_t st) {
-  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st);
-  else force_launch3<NS, ND, 1>(d, virial, block, st);
+static void force_launch2(const DevState& d, int env, bool virial, int block, hipStream_t st,
+                          double* vb) {
+  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st, vb);
+  else force_launch3<NS, ND, 1>(d, virial, block, st, vb);
 }
 
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st) {
+                  hipStream_t st, bool fuse) {
   if (!d.n) return;
-  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st);
-  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st);
-  else force_launch2<3, 1>(d, env, virial, block, st);
+  double* vb = fuse ? bead_vir_buffer(d) : nullptr;
+  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st, vb);
+  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st, vb);
+  else force_launch2<3, 1>(d, env, virial, block, st, vb);
 }
 
 // ---------------------------------------------------------------- pair listing

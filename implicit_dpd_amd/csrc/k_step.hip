@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "saru.hpp"
 #include "state.hpp"
+#include "bonded.hpp"
 
 namespace mdpd {
 
@@ -165,7 +166,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& excl) 
   return tot;
 }
 
-__global__ void __launch_bounds__(256) k_scan1(const uint32_t* __restrict__ in,
+// The histogram is cleared as it is read, ready for the next rebuild.
+__global__ void __launch_bounds__(256) k_scan1(uint32_t* __restrict__ in,
                                                uint32_t* __restrict__ out,
                                                uint32_t* __restrict__ bsum, uint32_t m) {
   const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
@@ -173,6 +175,7 @@ __global__ void __launch_bounds__(256) k_scan1(const uint32_t* __restrict__ in,
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     v[k] = (base + k < m) ? in[base + k] : 0u;
+    if (base + k < m) in[base + k] = 0u;
     t += v[k];
   }
   uint32_t excl;
@@ -207,12 +210,26 @@ __global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint
   }
 }
 
+// carry: bsum holds block totals and every block sums those before it
+// (few blocks), else bsum already holds the exclusive block offsets
 __global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
                                                uint32_t* __restrict__ cursor,
                                                const uint32_t* __restrict__ bsum, uint32_t m,
-                                               uint32_t total) {
+                                               uint32_t total, int carry) {
   const uint32_t base = blockIdx.x * 1024u;
-  const uint32_t add = bsum[blockIdx.x];
+  uint32_t add;
+  if (carry) {
+    __shared__ uint32_t part[4];
+    uint32_t v = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256u) v += bsum[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    add = part[0] + part[1] + part[2] + part[3];
+  } else {
+    add = bsum[blockIdx.x];
+  }
   for (uint32_t k = threadIdx.x; k < 1024u; k += 256u) {
     const uint32_t idx = base + k;
     if (idx < m) {
@@ -228,8 +245,10 @@ void launch_scan_cells(const DevState& d, hipStream_t st) {
   const uint32_t m = d.ncell;
   const uint32_t nb = div_up(m, 1024u);
   hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, st, d.cell_cnt, d.cell_start, d.bsum, m);
-  hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb);
-  hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, st, d.cell_start, d.cursor, d.bsum, m, d.n);
+  const int carry = nb <= 4096u ? 1 : 0;  // up to 4M cells: no separate block-sum pass
+  if (!carry) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb);
+  hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, st, d.cell_start, d.cursor, d.bsum, m, d.n,
+                     carry);
 }
 
 __global__ void __launch_bounds__(256) k_scatter(DevState d) {
@@ -283,9 +302,11 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
   const uint32_t cap = d.cap;
   float l[3], v[3];
+  double xs[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double x = d.pos[a * cap + i];
+    xs[a] = x;
     const double vb = d.vbuf[a * cap + i];
     d.alt.pos[a * cap + s] = x;
     d.alt.vbuf[a * cap + s] = vb;
@@ -304,7 +325,9 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   d.alt.amp_id[s] = d.amp_id[i];
   d.alt.unit[s] = d.unit[i];
   d.alt.orig[s] = d.orig[i];
-  d.alt.chain[s] = d.chain[i];
+  const uint32_t ch = d.chain[i];
+  d.alt.chain[s] = ch;
+  if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(xs[0], xs[1], xs[2], 0.0);
   d.spos[s] = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
   d.svel[s] = make_float4(v[0], v[1], v[2], 0.f);
   d.scell[s] = key;
@@ -324,6 +347,18 @@ __global__ void __launch_bounds__(256) k_topo_remap(DevState d) {
 void launch_topo_remap(const DevState& d, hipStream_t st) {
   if (!d.n_amp) return;
   hipLaunchKernelGGL(k_topo_remap, grid1(4 * d.n_amp, 256), dim3(256), 0, st, d);
+}
+
+// the topology from the chain map (every chain bead knows its chain slot)
+__global__ void __launch_bounds__(256) k_topo_from_chain(DevState d) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const uint32_t ch = d.chain[s];
+  if (ch != 0xffffffffu) d.topo[ch] = s;
+}
+void launch_topo_from_chain(const DevState& d, hipStream_t st) {
+  if (!d.n || !d.n_amp) return;
+  hipLaunchKernelGGL(k_topo_from_chain, grid1(d.n, 256), dim3(256), 0, st, d);
 }
 
 // ---------------------------------------------------------------- kick
@@ -508,117 +543,33 @@ void launch_observe(const DevState& d, const uint32_t* amp_num, uint32_t* inv_or
 }
 
 // ---------------------------------------------------------------- bonded
-// ForceBonded::CalcBondBendGlobalCell<4>, bdf_calculator.hpp:153-189, with
-// MinImage (:55-59), StoreBondForceWithARLaw (:95-115) and
-// StoreBendForceWithARLaw (:117-150), FP64, one thread per chain.  Chains
-// are disjoint, so the read-modify-write of acc needs no atomics.
-__device__ __forceinline__ void bd_min_image(const Replica& R, double d[3]) {
-#pragma clang fp contract(off)
-  for (int a = 0; a < 3; ++a) d[a] -= R.box[a] * round(d[a] * R.inv_box[a]);
-}
-
-
-// Bonded terms of chain a (ForceBonded::CalcBondBendGlobalCell<4>,
-// bdf_calculator.hpp:153-189, with MinImage :55-59, the bond :95-115 and
-// the bend :117-150), FP64 in the reference's order: F[u] receives its terms
-// in exactly the sequence of the reference loop.
-__device__ __forceinline__ void chain_terms(const DevState& d, const Replica& R, uint32_t a,
-                                            double F[4][3], double vir[3]) {
-#pragma clang fp contract(off)
-  const uint32_t cap = d.cap;
-  uint32_t l[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) l[u] = d.topo[4 * a + u];
-  const double cf_b = d.cf_bend[a];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) F[u][c] = 0.0;
-  double dr[3][3], inv_dr[3];
-  vir[0] = vir[1] = vir[2] = 0.0;
-#pragma unroll
-  for (int u = 1; u < 4; ++u) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) dr[u - 1][c] = d.pos[c * cap + l[u]] - d.pos[c * cap + l[u - 1]];
-    bd_min_image(R, dr[u - 1]);
-    const double dist2 = dr[u - 1][0] * dr[u - 1][0] + dr[u - 1][1] * dr[u - 1][1] +
-                         dr[u - 1][2] * dr[u - 1][2];
-    inv_dr[u - 1] = 1.0 / sqrt(dist2);
-    const double cf_bond = R.cf_s * (inv_dr[u - 1] - R.ibond);
-    double Fb[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Fb[c] = cf_bond * dr[u - 1][c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) vir[c] += 2.0 * dr[u - 1][c] * Fb[c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      F[u - 1][c] -= Fb[c];
-      F[u][c] += Fb[c];
-    }
-    if (u >= 2) {
-      const double* d0 = dr[u - 2];
-      const double* d1 = dr[u - 1];
-      const double inv_dr_prod = inv_dr[u - 2] * inv_dr[u - 1];
-      const double inv_dist0 = inv_dr[u - 2] * inv_dr[u - 2];
-      const double inv_dist1 = inv_dr[u - 1] * inv_dr[u - 1];
-      const double in_prod = d0[0] * d1[0] + d0[1] * d1[1] + d0[2] * d1[2];
-      const double cf_bd = cf_b * inv_dr_prod;
-      const double crs0 = in_prod * inv_dist0, crs1 = in_prod * inv_dist1;
-      double T0[3], T1[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T0[c] = cf_bd * (d1[c] - crs0 * d0[c]);
-        T1[c] = cf_bd * (d0[c] - crs1 * d1[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) vir[c] += 2.0 * (d0[c] * T0[c] + d1[c] * T1[c]);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        F[u - 2][c] -= T0[c];
-        F[u - 1][c] += T0[c] - T1[c];
-        F[u][c] += T1[c];
-      }
-    }
-  }
-}
-
 // One thread per bead: the bead evaluates its own chain (four beads, a few
 // hundred FP64 flops) and keeps its own row, so every write is to the bead's
 // own slot; then kick (driftkick.hpp:32-40) from the same registers.  The
 // chain virial is counted once, by the chain's first bead.
 __global__ void __launch_bounds__(256) k_bonded_kick(DevState d, int virial, int kick,
                                                      double* __restrict__ vir_bead) {
-#pragma clang fp contract(off)
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
-  const Replica& R = d.reps[r];
-  const uint32_t cap = d.cap;
-  const uint32_t ch = d.chain[s];
-  double vir[3] = {0.0, 0.0, 0.0};
   float af[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) af[c] = d.acc[c * cap + s];
-  if (ch != 0xffffffffu) {
-    const uint32_t u = ch & 3u;
-    double F[4][3];
-    chain_terms(d, R, ch >> 2, F, vir);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      af[c] = (float)((double)af[c] + F[u][c]);
-      d.acc[c * cap + s] = af[c];
-    }
-    if (u != 0) vir[0] = vir[1] = vir[2] = 0.0;
-  }
-  if (virial)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) vir_bead[3 * (size_t)s + c] = vir[c];
-  if (kick) {
-    const double dt = R.dt;
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      d.vel[c * cap + s] = d.vbuf[c * cap + s] + dt * 0.5 * (double)af[c];
-  }
+  for (int c = 0; c < 3; ++c) af[c] = d.acc[c * d.cap + s];
+  bonded_kick_bead(d, d.reps[r], s, af, virial != 0, kick != 0, vir_bead);
+}
+
+// chain-major FP64 positions of every chain bead from the master storage
+// (the step's permutation writes them as it moves the beads)
+__global__ void __launch_bounds__(256) k_chainpos_fill(DevState d) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const uint32_t ch = d.chain[s];
+  if (ch != 0xffffffffu)
+    d.chainpos[ch] = make_double4(d.pos[s], d.pos[d.cap + s], d.pos[2 * (size_t)d.cap + s], 0.0);
+}
+void launch_chainpos_fill(const DevState& d, hipStream_t st) {
+  if (!d.n || !d.n_amp) return;
+  hipLaunchKernelGGL(k_chainpos_fill, grid1(d.n, 256), dim3(256), 0, st, d);
 }
 
 // per-replica sum of the per-bead chain virials
@@ -633,22 +584,27 @@ __global__ void __launch_bounds__(kRedThreads) k_bead_vir(DevState d, const doub
   block_reduce_write<3>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 6);
 }
 
+double* bead_vir_buffer(const DevState& d) {
+  return d.red + (size_t)d.n_rep * kRedBlocks * 2 * 16;
+}
+
+void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st) {
+  if (d.n_amp) {
+    hipLaunchKernelGGL(k_bead_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d,
+                       (const double*)bead_vir_buffer(d), d.red);
+    hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out);
+  } else {
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
+  }
+}
+
 void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_out,
                         hipStream_t st) {
   if (!d.n) return;
-  double* vir_bead = d.red + (size_t)d.n_rep * kRedBlocks * 2 * 16;
   if (d.n_amp || kick)
     hipLaunchKernelGGL(k_bonded_kick, grid1(d.n, 256), dim3(256), 0, st, d, virial ? 1 : 0,
-                       kick ? 1 : 0, vir_bead);
-  if (virial) {
-    if (d.n_amp) {
-      hipLaunchKernelGGL(k_bead_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d,
-                         (const double*)vir_bead, d.red);
-      hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out);
-    } else {
-      hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
-    }
-  }
+                       kick ? 1 : 0, bead_vir_buffer(d));
+  if (virial) launch_bead_vir_reduce(d, vir_out, st);
 }
 
 // ---------------------------------------------------------------- reactions
@@ -856,13 +812,16 @@ __global__ void k_chem_insert(DevState d, const uint32_t* __restrict__ targets, 
   l = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
   for (int a = 0; a < 3; ++a) tv[a] /= l;
   const uint32_t i = d.n + g;
+  double np[3];
   for (int a = 0; a < 3; ++a) {
     tv[a] *= eps;
     h2e[a] *= bond_leng;
     double x = (tv[a] + d.pos[a * cap + hd]) + h2e[a] * (double)u;
     x -= floor(x * R.inv_box[a]) * R.box[a];
     d.pos[a * cap + i] = x;
+    np[a] = x;
   }
+  d.chainpos[4 * (size_t)new_amp + u] = make_double4(np[0], np[1], np[2], 0.0);
   Saru sv = saru_seed(i, time, R.chem_seed ^ TAG_VEL);
   for (int a = 0; a < 3; ++a) {
     const double u1 = saru_uni_f64(sv);

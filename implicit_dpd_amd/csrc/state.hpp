@@ -52,6 +52,7 @@ struct DevState : Master {
   int32_t n_dens;
   // bonded
   uint32_t* topo;       // [4][amp_cap] chain bead slots, amp-major
+  double4* chainpos;    // [4 amp_cap] FP64 positions of chain beads, chain-major
   double* cf_bend;      // [amp_cap]
   uint32_t* amp_rep;    // [amp_cap] replica of each chain
   uint32_t* amp_off;    // [n_rep + 1] first chain of each replica
@@ -74,14 +75,22 @@ void launch_cell_sort(const DevState& d, hipStream_t st);
 // copies; `full` also carries vel, acc and press (needed when they are live)
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st);
 void launch_topo_remap(const DevState& d, hipStream_t st);
+// topo rebuilt from chain[] (the step keeps only chain[] current)
+void launch_topo_from_chain(const DevState& d, hipStream_t st);
 void launch_density(const DevState& d, int ns, int nd, int block, hipStream_t st);
+// fuse: the force pass ends with the bead's bonded terms and the kick
+// (virial steps: per-bead chain virials for launch_bead_vir_reduce)
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st);
+                  hipStream_t st, bool fuse = false);
+double* bead_vir_buffer(const DevState& d);
+void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st);
 void launch_kick(const DevState& d, hipStream_t st);
 // bonded forces of every bead's own chain added to its acceleration, then
 // the kick, in one pass over the beads (chain virial on virial steps)
 void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_out,
                         hipStream_t st);
+// chainpos from the master positions (the permutation keeps it current)
+void launch_chainpos_fill(const DevState& d, hipStream_t st);
 void launch_cm_drift(const DevState& d, hipStream_t st);
 void launch_head_conversion(const DevState& d, hipStream_t st);
 void launch_time_inc(uint32_t* d_time, hipStream_t st);
