@@ -134,12 +134,19 @@ def ensemble_samples(n_launch, n_rep=64, nmin=4096, nmax=65536, steps=3000, ever
 
 
 def sweep(plain, target=TARGET_RHO, lo=A_RANGE[0], hi=A_RANGE[1], max_runs=12):
-    """Regula falsi on A with plain runs; rho decreases as A rises."""
+    """Regula falsi (Illinois) on A with plain runs; rho decreases as A
+    rises.  Returns (A, runs); A is None when [lo, hi] does not bracket the
+    target."""
     runs = []
     f_lo = plain(lo)[0] - target
     runs.append((lo, f_lo + target))
     f_hi = plain(hi)[0] - target
     runs.append((hi, f_hi + target))
+    for a, rho in runs:
+        if abs(rho - target) <= TOL * target:
+            return a, runs
+    if (f_lo > 0) == (f_hi > 0):
+        return None, runs
     a = None
     while len(runs) < max_runs:
         a = hi - f_hi * (hi - lo) / (f_hi - f_lo)
@@ -154,32 +161,47 @@ def sweep(plain, target=TARGET_RHO, lo=A_RANGE[0], hi=A_RANGE[1], max_runs=12):
         else:
             hi, f_hi = a, f
             f_lo *= 0.5
-    return a, runs
+    return None, runs
 
 
-def surrogate_calibrate(X, Y, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6, seed=3,
-                        epochs=300):
+def train_surrogate(X, Y, epochs=300, seed=3):
     xs, ys = Scaler(X), Scaler(Y)
     net = Surrogate(max_batch=4096, seed=seed)
     hist = net.train(xs.fwd(X), ys.fwd(Y), epochs=epochs, batch=256, lr=2e-3, seed=seed)
+    return net, xs, ys, hist
+
+
+def surrogate_calibrate(net, xs, ys, X, Y, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6,
+                        seed=3):
+    """Proposes A from the network (scan of A at fixed B, rd, kBT = 1), checks
+    it with a plain run, and on a miss adds the checked point to the data and
+    fine-tunes.  The network is restored to its trained state afterwards."""
+    base = net.params
     grid = np.linspace(A_RANGE[0], A_RANGE[1], 4001)
     cand = np.stack([grid, np.full_like(grid, B), np.full_like(grid, rd), np.ones_like(grid)], 1)
     runs = []
     Xa, Ya = X.copy(), Y.copy()
-    a = None
+    a, hit = None, False
     for _ in range(max_runs):
         pred = ys.inv(net.forward(xs.fwd(cand)))
-        a = float(grid[np.argmin(np.abs(pred[:, 0] - target))])
+        i = int(np.argmin(np.abs(pred[:, 0] - target)))
+        a = float(grid[i])
         rho, pzz = plain(a)
-        runs.append((a, rho, float(pred[np.argmin(np.abs(grid - a)), 0])))
+        runs.append((a, rho, float(pred[i, 0])))
         if abs(rho - target) <= TOL * target:
+            hit = True
             break
-        # add the verified point (weighted) and fine-tune
         Xa = np.concatenate([Xa, np.repeat([[a, B, rd, 1.0]], 64, 0)])
         Ya = np.concatenate([Ya, np.repeat([[rho, pzz]], 64, 0)])
         net.train(xs.fwd(Xa), ys.fwd(Ya), epochs=40, batch=256, lr=1e-3, seed=seed + len(runs))
-    net.close()
-    return a, runs, hist
+    net.params = base
+    return (a if hit else None), runs
+
+
+# (target bulk density, B, rd): the headline case first, then further
+# targets inside the sampled ranges that show how the sample set amortises
+MORE_TARGETS = [(5.5, 25.0, 0.75), (6.5, 25.0, 0.75), (6.0, 20.0, 0.70), (6.0, 35.0, 0.80),
+                (5.0, 30.0, 0.70), (7.0, 30.0, 0.80)]
 
 
 def main(argv=None):
@@ -190,35 +212,50 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--plain-beads", type=int, default=200000)
     ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--more-targets", action="store_true",
+                    help="also calibrate six further (rho, B, rd) targets")
     args = ap.parse_args(argv)
 
-    gpu_s = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0}
+    secs = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0}
 
-    def plain(a, key):
-        t0 = time.perf_counter()
-        r = plain_run(a, n=args.plain_beads, steps=args.steps)
-        gpu_s[key] += time.perf_counter() - t0
-        log(f"[calib] plain run A={a:.4f}: rho={r[0]:.4f} P_zz={r[1]:.4f}")
-        return r
+    def plain(key, B, rd):
+        def run(a):
+            t0 = time.perf_counter()
+            r = plain_run(a, B=B, rd=rd, n=args.plain_beads, steps=args.steps)
+            secs[key] += time.perf_counter() - t0
+            log(f"[calib] plain run A={a:.4f} B={B} rd={rd}: rho={r[0]:.4f} P_zz={r[1]:.4f}")
+            return r
+        return run
 
     t0 = time.perf_counter()
     X, Y = ensemble_samples(args.launches, nmin=args.nmin, nmax=args.nmax, steps=args.steps)
-    gpu_s["samples"] = time.perf_counter() - t0
+    secs["samples"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    a_s, runs_s, hist = surrogate_calibrate(X, Y, lambda a: plain(a, "surrogate_runs"),
-                                            epochs=args.epochs)
-    gpu_s["train"] = time.perf_counter() - t0 - gpu_s["surrogate_runs"]
-    a_p, runs_p = sweep(lambda a: plain(a, "sweep_runs"))
+    net, xs, ys, hist = train_surrogate(X, Y, epochs=args.epochs)
+    secs["train"] = time.perf_counter() - t0
+    targets = [(TARGET_RHO, 25.0, 0.75)] + (MORE_TARGETS if args.more_targets else [])
+    results = []
+    for rho_t, B, rd in targets:
+        a_s, runs_s = surrogate_calibrate(net, xs, ys, X, Y, plain("surrogate_runs", B, rd),
+                                          target=rho_t, B=B, rd=rd)
+        a_p, runs_p = sweep(plain("sweep_runs", B, rd), target=rho_t)
+        results.append({
+            "target": {"rho": rho_t, "tol": TOL, "B": B, "rd": rd, "kBT": 1.0},
+            "surrogate": {"A": a_s, "plain_runs": len(runs_s),
+                          "runs": [{"A": a, "rho": r, "rho_pred": q} for a, r, q in runs_s]},
+            "sweep": {"A": a_p, "plain_runs": len(runs_p),
+                      "runs": [{"A": a, "rho": r} for a, r in runs_p]}})
+    net.close()
+    n_s = sum(r["surrogate"]["plain_runs"] for r in results)
+    n_p = sum(r["sweep"]["plain_runs"] for r in results)
     out = {
-        "target": {"rho": TARGET_RHO, "tol": TOL, "B": 25.0, "rd": 0.75, "kBT": 1.0},
         "samples": int(len(X)),
         "train_loss_first_last": [float(hist[0]), float(hist[-1])],
-        "surrogate": {"A": a_s, "plain_runs": len(runs_s),
-                      "runs": [{"A": a, "rho": r, "rho_pred": q} for a, r, q in runs_s]},
-        "sweep": {"A": a_p, "plain_runs": len(runs_p),
-                  "runs": [{"A": a, "rho": r} for a, r in runs_p]},
-        "plain_run_ratio": len(runs_p) / max(len(runs_s), 1),
-        "seconds": {k: round(v, 2) for k, v in gpu_s.items()},
+        "targets": results,
+        "plain_runs": {"surrogate": n_s, "sweep": n_p, "ratio": round(n_p / max(n_s, 1), 3)},
+        "hits": {"surrogate": sum(r["surrogate"]["A"] is not None for r in results),
+                 "sweep": sum(r["sweep"]["A"] is not None for r in results)},
+        "seconds": {k: round(v, 2) for k, v in secs.items()},
     }
     print(json.dumps(out), flush=True)
     return out
