@@ -132,9 +132,10 @@ typedef struct mdpd_observables {
 typedef struct mdpd_tuning {
   int32_t block_scan;       /* threads per block, density/pair-list pass */
   int32_t block_force;      /* threads per block, force pass */
-  int32_t cell_div;         /* cells per unit cutoff along each axis (1 or 2) */
+  int32_t cell_div;         /* cells per unit cutoff along each axis; must be 1 */
   int32_t nlist_width;      /* pair-list slots per bead, a multiple of 4 */
-  int32_t reorder_every;    /* steps between spatial reorders of bead storage (0: never) */
+  int32_t reorder_every;    /* ignored: bead storage is put in cell order at
+                               every cell-list rebuild */
   int32_t use_graph;        /* capture each step in a hipGraph */
 } mdpd_tuning;
 
