@@ -174,6 +174,7 @@ PROTOTYPES = {
     "mdpd_autotune": (C.c_int, [VP, C.c_uint32, P(MdpdTuning)]),
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
+    "mdpd_run_timed": (C.c_int, [VP, C.c_uint32, _DP]),
     # equation-of-state surrogate
     "mdpd_mlp_num_params": (C.c_int, [_U32P]),
     "mdpd_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, P(VP)]),

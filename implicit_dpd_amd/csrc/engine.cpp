@@ -826,6 +826,29 @@ int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
   return check_flags(c);
 }
 
+int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
+  if (!c || !gpu_ms) return MDPD_ERR_ARG;
+  if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipEventRecord(e0, c->st));
+  int rc = run_steps(c, nsteps);
+  if (rc == MDPD_OK) {
+    HIPCHK(c, hipEventRecord(e1, c->st));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *gpu_ms = ms;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  return check_flags(c);
+}
+
 int mdpd_sync(mdpd_ctx* c) {
   if (!c) return MDPD_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->st));

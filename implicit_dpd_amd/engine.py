@@ -184,6 +184,12 @@ class Engine:
     def run(self, nsteps):
         self._c(self.L.mdpd_run(self.h, int(nsteps)))
 
+    def run_timed(self, nsteps):
+        """Runs nsteps (captured graphs) and returns the device time in ms."""
+        ms = C.c_double()
+        self._c(self.L.mdpd_run_timed(self.h, int(nsteps), C.byref(ms)))
+        return ms.value
+
     def sync(self):
         self._c(self.L.mdpd_sync(self.h))
 

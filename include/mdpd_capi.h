@@ -205,6 +205,9 @@ int mdpd_density_profile(mdpd_ctx* ctx, int32_t axis, int32_t nbins,
 /* `nsteps` iterations of main.cpp:166-196 (drift .. time++); the pair
    virial of the last step is kept for mdpd_observe. */
 int mdpd_run(mdpd_ctx* ctx, uint32_t nsteps);
+/* mdpd_run with HIP events recorded on the context's stream around the
+   steps; *gpu_ms is the device time between them.  Used by the benchmark. */
+int mdpd_run_timed(mdpd_ctx* ctx, uint32_t nsteps, double* gpu_ms);
 /* Blocks until the context's stream is idle. */
 int mdpd_sync(mdpd_ctx* ctx);
 
