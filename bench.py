@@ -167,8 +167,14 @@ def main():
     log(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step wall, "
         f"{gpu_ms / args.steps:.4f} ms/step device")
 
-    # the next K steps once more, launched kernel by kernel with HIP events
-    # around every phase: the per-kernel times of the roofline
+    # the same K steps once more from the same start (the trajectory is
+    # deterministic), launched kernel by kernel with HIP events around every
+    # phase: the per-kernel times of the roofline
+    obs_timed = e.observe()
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(0, p.pair_seed)
+    e.run(max(args.warmup, 1))
     e.set_profiling(True)
     e.run(args.steps)
     e.sync()
@@ -176,6 +182,8 @@ def main():
     e.set_profiling(False)
     log("[bench] phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in ph.items()))
     obs = e.observe()
+    if list(obs["kin_temp"]) != list(obs_timed["kin_temp"]):
+        log("[bench] note: the replayed steps did not reproduce the timed ones")
     n_pairs = obs["n_pairs"]
     nd = p.n_dens
 
