@@ -12,9 +12,9 @@ P_zz = 0 (the coexistence condition).
 - The *sweep* finds A by regula falsi on plain runs alone.
 - The *surrogate* is trained on ensemble runs (64 replicas of U[4k, 64k]
   beads per launch with A ~ U[-60,-20], B ~ U[10,40], rd ~ U[0.65,0.85],
-  kBT = 1), proposes A by scanning the network, and each proposal is
-  verified by a plain run; a miss is added to the training set and the
-  network fine-tuned.
+  kBT = 1) and proposes A by scanning the network; each proposal is verified
+  by a plain run, and a miss is followed by a Newton step on the measured
+  residual with the network's slope (the secant once two runs exist).
 
 The headline of `main()` is the number of plain runs each method needs.
 """
@@ -171,31 +171,45 @@ def train_surrogate(X, Y, epochs=300, seed=3):
     return net, xs, ys, hist
 
 
-def surrogate_calibrate(net, xs, ys, X, Y, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6,
-                        seed=3):
-    """Proposes A from the network (scan of A at fixed B, rd, kBT = 1), checks
-    it with a plain run, and on a miss adds the checked point to the data and
-    fine-tunes.  The network is restored to its trained state afterwards."""
-    base = net.params
+def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6,
+                        reach=0.05):
+    """Surrogate-guided root finding on A at fixed B, rd, kBT = 1.
+
+    The network is scanned over A for the first proposal.  A target the
+    network puts more than `reach` (relative) away from every A of the range
+    is checked once at the closest A and reported unreachable.  After a miss
+    the next A is a Newton step on the measured residual, with the slope of
+    the network at the current A (and the secant of the last two plain runs
+    once there are two)."""
     grid = np.linspace(A_RANGE[0], A_RANGE[1], 4001)
     cand = np.stack([grid, np.full_like(grid, B), np.full_like(grid, rd), np.ones_like(grid)], 1)
+    pred = ys.inv(net.forward(xs.fwd(cand)))[:, 0]
+    i = int(np.argmin(np.abs(pred - target)))
+    a = float(grid[i])
     runs = []
-    Xa, Ya = X.copy(), Y.copy()
-    a, hit = None, False
+
+    def slope_at(x):
+        j = int(np.clip(np.searchsorted(grid, x), 40, len(grid) - 41))
+        return (pred[j + 40] - pred[j - 40]) / (grid[j + 40] - grid[j - 40])
+
+    if abs(pred[i] - target) > reach * target:
+        rho, _ = plain(a)
+        runs.append((a, rho, float(pred[i])))
+        return None, runs
     for _ in range(max_runs):
-        pred = ys.inv(net.forward(xs.fwd(cand)))
-        i = int(np.argmin(np.abs(pred[:, 0] - target)))
-        a = float(grid[i])
-        rho, pzz = plain(a)
-        runs.append((a, rho, float(pred[i, 0])))
+        rho, _ = plain(a)
+        runs.append((a, rho, float(np.interp(a, grid, pred))))
         if abs(rho - target) <= TOL * target:
-            hit = True
+            return a, runs
+        sl = slope_at(a)
+        if len(runs) >= 2 and abs(runs[-1][0] - runs[-2][0]) > 1e-6:
+            sec = (runs[-1][1] - runs[-2][1]) / (runs[-1][0] - runs[-2][0])
+            if sec * sl > 0:
+                sl = sec
+        if sl == 0 or not np.isfinite(sl):
             break
-        Xa = np.concatenate([Xa, np.repeat([[a, B, rd, 1.0]], 64, 0)])
-        Ya = np.concatenate([Ya, np.repeat([[rho, pzz]], 64, 0)])
-        net.train(xs.fwd(Xa), ys.fwd(Ya), epochs=40, batch=256, lr=1e-3, seed=seed + len(runs))
-    net.params = base
-    return (a if hit else None), runs
+        a = float(np.clip(a + (target - rho) / sl, A_RANGE[0], A_RANGE[1]))
+    return None, runs
 
 
 # (target bulk density, B, rd): the headline case first, then further
@@ -214,6 +228,9 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--more-targets", action="store_true",
                     help="also calibrate six further (rho, B, rd) targets")
+    ap.add_argument("--save-samples", default=None, help="write the ensemble samples (.npz)")
+    ap.add_argument("--load-samples", default=None,
+                    help="reuse ensemble samples written by --save-samples")
     args = ap.parse_args(argv)
 
     secs = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0}
@@ -227,16 +244,23 @@ def main(argv=None):
             return r
         return run
 
-    t0 = time.perf_counter()
-    X, Y = ensemble_samples(args.launches, nmin=args.nmin, nmax=args.nmax, steps=args.steps)
-    secs["samples"] = time.perf_counter() - t0
+    if args.load_samples:
+        z = np.load(args.load_samples)
+        X, Y = z["X"], z["Y"]
+        secs["samples"] = float(z["seconds"])  # GPU time of the run that made them
+    else:
+        t0 = time.perf_counter()
+        X, Y = ensemble_samples(args.launches, nmin=args.nmin, nmax=args.nmax, steps=args.steps)
+        secs["samples"] = time.perf_counter() - t0
+    if args.save_samples:
+        np.savez(args.save_samples, X=X, Y=Y, seconds=secs["samples"])
     t0 = time.perf_counter()
     net, xs, ys, hist = train_surrogate(X, Y, epochs=args.epochs)
     secs["train"] = time.perf_counter() - t0
     targets = [(TARGET_RHO, 25.0, 0.75)] + (MORE_TARGETS if args.more_targets else [])
     results = []
     for rho_t, B, rd in targets:
-        a_s, runs_s = surrogate_calibrate(net, xs, ys, X, Y, plain("surrogate_runs", B, rd),
+        a_s, runs_s = surrogate_calibrate(net, xs, ys, plain("surrogate_runs", B, rd),
                                           target=rho_t, B=B, rd=rd)
         a_p, runs_p = sweep(plain("sweep_runs", B, rd), target=rho_t)
         results.append({

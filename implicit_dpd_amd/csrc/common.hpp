@@ -42,6 +42,7 @@ struct Replica {
   float rd, inv_rd, crho, inv_rdm, arc;
   float rd2f;
   int32_t rdm_is_one;
+  int32_t has_c1;      // any c1 != 0 (the c1 * max(r - arc, 0) term is live)
   float a0[3][3], c1[3][3], sigma[3][3], gamma[3][3];
   float mb[3][3][3];
 };

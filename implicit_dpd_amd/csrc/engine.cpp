@@ -118,6 +118,8 @@ Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off,
   r.arc = (float)p.arc;
   r.rd2f = (float)(p.rd * p.rd);
   r.rdm_is_one = (p.rdm == 1.0) ? 1 : 0;
+  r.has_c1 = 0;
+  for (int i = 0; i < 9; ++i) r.has_c1 |= (p.c1[i / 3][i % 3] != 0.0) ? 1 : 0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       r.a0[i][j] = (float)p.a0[i][j];

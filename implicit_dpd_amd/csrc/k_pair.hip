@@ -192,8 +192,26 @@ __device__ __forceinline__ float sel(const float (&v)[NS], uint32_t t) {
   return t == 0u ? v[0] : (t == 1u ? v[1] : v[2]);
 }
 
+// Pair coefficients of one system by type pair (3 ti + tj):
+// [0] = (a0, sigma, gamma, mb[0]), [1] = (c1, mb[1], mb[2], 0).
+__device__ __forceinline__ void fill_pair_table(const Replica& R, float4* tab) {
+  const uint32_t k = threadIdx.x;
+  if (k < 9u) {
+    const uint32_t a = k / 3u, b = k % 3u;
+    tab[2 * k] = make_float4(R.a0[a][b], R.sigma[a][b], R.gamma[a][b], R.mb[a][b][0]);
+    tab[2 * k + 1] = make_float4(R.c1[a][b], R.mb[a][b][1], R.mb[a][b][2], 0.0f);
+  }
+}
+
 template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
 __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead) {
+  // single system: the coefficients come from a block-shared table indexed
+  // by the type pair instead of per-lane selects
+  __shared__ float4 ptab[18];
+  if (!MULTI && NS > 1) {
+    fill_pair_table(d.reps[0], ptab);
+    __syncthreads();
+  }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -202,7 +220,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   const float4 pi = d.spos[s];
   const float4 vi = d.svel[s];
   const uint32_t wi = __float_as_uint(pi.w);
-  const uint32_t idi = wi >> 2, ti = NS == 1 ? 0u : (wi & 3u);
+  const uint32_t idi = wi >> 2, ti = NS == 1 ? 0u : (wi & 3u), ti3 = 3u * ti;
   float rho_i[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho_i[k] = d.sdens[(size_t)k * d.cap + s];
@@ -219,6 +237,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
   const float arc = R.arc, inv_rdm = R.inv_rdm;
   const bool rdm1 = R.rdm_is_one != 0;
+  const bool has_c1 = R.has_c1 != 0;
   const uint32_t W = (uint32_t)d.W;
   uint32_t cnt = d.ncount[s];
   cnt = cnt < W ? cnt : W;
@@ -249,14 +268,38 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     float wR, wD;
     if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
     else { wR = wc; wD = wc * wc; }
-    const float cons_c = sel<NS>(A0, tj) + sel<NS>(C1, tj) * fmaxf(rr - arc, 0.0f);
+    float a0, c1, sg, gm, mbk[ND];
+    if (!MULTI && NS > 1) {
+      const float4 q0 = ptab[2 * (ti3 + tj)];
+      a0 = q0.x;
+      sg = q0.y;
+      gm = q0.z;
+      mbk[0] = q0.w;
+      if (ND > 1 || has_c1) {
+        const float4 q1 = ptab[2 * (ti3 + tj) + 1];
+        c1 = q1.x;
+        if (ND > 1) {
+          mbk[1 % ND] = q1.y;
+          mbk[2 % ND] = q1.z;
+        }
+      } else {
+        c1 = 0.0f;
+      }
+    } else {
+      a0 = sel<NS>(A0, tj);
+      c1 = sel<NS>(C1, tj);
+      sg = sel<NS>(SG, tj);
+      gm = sel<NS>(GM, tj);
+#pragma unroll
+      for (int k = 0; k < ND; ++k) mbk[k] = sel<NS>(MB[k], tj);
+    }
+    const float cons_c = has_c1 ? a0 + c1 * fmaxf(rr - arc, 0.0f) : a0;
     float mbs = 0.f;
 #pragma unroll
-    for (int k = 0; k < ND; ++k) mbs += sel<NS>(MB[k], tj) * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
+    for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
     const float wm = rdm1 ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
     const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
-    const float fs = (cons_c * wc + mbs * wm + sel<NS>(SG, tj) * wR * xi -
-                      sel<NS>(GM, tj) * wD * rv * rinv) * rinv;
+    const float fs = (cons_c * wc + mbs * wm + sg * wR * xi - gm * wD * rv * rinv) * rinv;
     fx += fs * dx;
     fy += fs * dy;
     fz += fs * dz;
