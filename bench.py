@@ -111,11 +111,73 @@ def cpu_baseline(p, s, topo, budget_s, max_steps):
                       f"system, {steps} steps after 1 warm-up step, {dt:.1f} s"}
 
 
+class Workload:
+    """One benchmark system: parameters, initial state and its description."""
+
+    def __init__(self, name, params, systems, text, bonded):
+        self.name, self.params, self.systems, self.text = name, params, systems, text
+        self.bonded = bonded
+        self.ensemble = isinstance(params, list)
+        plist = params if self.ensemble else [params]
+        slist = systems if self.ensemble else [systems]
+        self.sizes = [len(x["pos"]) for x in slist]
+        self.n = sum(self.sizes)
+        self.volume = sum(q.box[0] * q.box[1] * q.box[2] for q in plist)
+        self.ncell = sum(max(int(math.floor(q.box[0])), 1) * max(int(math.floor(q.box[1])), 1) *
+                         max(int(math.floor(q.box[2])), 1) for q in plist)
+        self.nd = plist[0].n_dens
+
+    def start(self, Engine, e=None, warmup=1):
+        """Uploads the initial state (into `e`, or a new engine) and runs
+        the warm-up steps."""
+        from implicit_dpd_amd import configs
+        if e is None:
+            e = (Engine(self.params, n_beads=self.sizes) if self.ensemble
+                 else Engine(self.params, capacity=self.n))
+            if self.ensemble:
+                e.set_tuning(nlist_width=256)  # dense corners of the sampled range
+        st = configs.concat_systems(self.systems) if self.ensemble else self.systems
+        e.upload(st["pos"], st["vel"], st["ids"], st["prop"], st["amp_id"], st["unit"])
+        if self.bonded:
+            e.set_topology(st["topo"])
+        e.set_clock(0, 0 if self.ensemble else self.params.pair_seed)
+        e.run(max(warmup, 1))
+        e.sync()
+        return e
+
+
+def workload(name, chains):
+    from implicit_dpd_amd import configs
+    if name == "membrane":
+        p, s = configs.membrane(n_chain=chains)
+        return Workload(name, p, s, f"membrane: {chains} H1T3 chains ({4 * chains} beads), box "
+                        "100x100x40, Warren MDPD A_ij in [-40,-20] by type, B=25, rd=0.75, bonds "
+                        "k=100 l0=0.5, bending 10, head conversion p=1e-4/step, dt=0.005", True)
+    if name == "slab":
+        p, s = configs.mdpd_slab()
+        return Workload(name, p, s, "MDPD slab: 198246 beads in the middle half of a 32x32x64 box, "
+                        "Warren A=-40, B=25, rd=0.75, gamma=4.5, kBT=1, dt=0.005", False)
+    if name == "groot_warren":
+        p, s = configs.groot_warren_fluid()
+        return Workload(name, p, s, "Groot-Warren DPD fluid: 24000 beads, L=20, rho=3, a=25, "
+                        "gamma=4.5, kBT=1, dt=0.01", False)
+    if name == "ensemble":
+        p, s = configs.ensemble()
+        return Workload(name, p, s, "replica ensemble: 64 slab replicas of U[4096,65536] beads "
+                        f"({sum(len(x['pos']) for x in s)} beads), A~U[-60,-20], B~U[10,40], "
+                        "rd~U[0.65,0.85], kBT=1, dt=0.005, one context", False)
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="membrane",
+                    choices=["membrane", "slab", "groot_warren", "ensemble"],
+                    help="membrane is the headline system; the others are the smaller "
+                         "configurations of the parity tests")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--chains", type=int, default=262144)
@@ -129,7 +191,6 @@ def main():
               "(spatial sharding with halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
 
-    from implicit_dpd_amd import configs
     from implicit_dpd_amd.engine import Engine, bandwidth_probe, device_name
 
     if args.calib_bytes:
@@ -138,22 +199,23 @@ def main():
                           "gbps": g}), flush=True)
         return
 
-    p, s = configs.membrane(n_chain=args.chains)
-    n = len(s["pos"])
-    box = [p.box[0], p.box[1], p.box[2]]
-    rho_mean = n / (box[0] * box[1] * box[2])
-    log(f"[bench] device {device_name()}; {n} beads, box {box}")
-    e = Engine(p, capacity=n)
-    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
-    e.set_topology(s["topo"])
-    e.set_clock(0, p.pair_seed)
+    w = workload(args.workload, args.chains)
+    n = w.n
+    rho_mean = n / w.volume
+    log(f"[bench] device {device_name()}; workload {w.name}: {n} beads")
     t = time.perf_counter()
-    e.run(max(args.warmup, 1))
-    e.sync()
+    e = w.start(Engine, warmup=args.warmup)
     log(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t:.2f} s")
     if args.autotune:
         tt = e.autotune(10)
         log(f"[bench] autotune: block_scan {tt.block_scan}, block_force {tt.block_force}")
+
+    def observe():
+        if not w.ensemble:
+            return e.observe()
+        obs = e.observe_replicas()
+        return {"kin_temp": [o["kin_temp"][3] for o in obs],
+                "n_pairs": sum(o["n_pairs"] for o in obs)}
 
     # timed region: K steps in production mode (captured graphs), the stream
     # idle on entry and synchronised on exit; HIP events on the engine stream
@@ -170,30 +232,24 @@ def main():
     # the same K steps once more from the same start (the trajectory is
     # deterministic), launched kernel by kernel with HIP events around every
     # phase: the per-kernel times of the roofline
-    obs_timed = e.observe()
-    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
-    e.set_topology(s["topo"])
-    e.set_clock(0, p.pair_seed)
-    e.run(max(args.warmup, 1))
+    obs_timed = observe()
+    w.start(Engine, e=e, warmup=args.warmup)
     e.set_profiling(True)
     e.run(args.steps)
     e.sync()
     ph = e.phase_times()
     e.set_profiling(False)
     log("[bench] phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in ph.items()))
-    obs = e.observe()
+    obs = observe()
     if list(obs["kin_temp"]) != list(obs_timed["kin_temp"]):
         log("[bench] note: the replayed steps did not reproduce the timed ones")
     n_pairs = obs["n_pairs"]
-    nd = p.n_dens
+    nd = w.nd
 
-    ncell = 1
-    for a in range(3):
-        ncell *= max(int(math.floor(box[a])), 1)
-    prof = profiled_counters(WORKLOAD)
+    prof = profiled_counters(WORKLOAD if w.name == "membrane" else f"{w.name}")
     kern = {}
     for name, phase, algo in (
-            ("k_density", "density", density_bytes_per_launch(n, n_pairs, nd, ncell)),
+            ("k_density", "density", density_bytes_per_launch(n, n_pairs, nd, w.ncell)),
             ("k_force", "force", force_bytes_per_launch(n, n_pairs, nd))):
         ms = ph[phase]
         gbs = algo / (ms * 1e-3) / 1e9
@@ -206,7 +262,7 @@ def main():
                       "profiled_variant": variant}
     dom = max(kern, key=lambda k: kern[k]["ms"])
     kd = kern[dom]
-    bstep = step_bytes_per_bead(nd, True, rho_mean)
+    bstep = step_bytes_per_bead(nd, w.bonded, rho_mean)
     roof = {
         "bound": "hbm",
         "achieved": kd["achieved_gbs"],
@@ -229,9 +285,10 @@ def main():
                 "frac": round(value * bstep / (HBM_PEAK_GBS * 1e9), 4)}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not w.ensemble:
         log("[bench] CPU baseline (oracle)...")
-        cpu = cpu_baseline(p, s, s["topo"], args.cpu_seconds, 400)
+        s = w.systems
+        cpu = cpu_baseline(w.params, s, s["topo"] if w.bonded else [], args.cpu_seconds, 400)
         log(f"[bench] CPU: {cpu['value']:.4g} bead-steps/s on {cpu['cores']} threads")
 
     out = {
@@ -248,9 +305,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "membrane: 262144 H1T3 chains (1048576 beads), box 100x100x40, Warren "
-                        "MDPD A_ij in [-40,-20] by type, B=25, rd=0.75, bonds k=100 l0=0.5, "
-                        "bending 10, head conversion p=1e-4/step, dt=0.005",
+            "workload": w.text,
             "beads": n,
             "precision": "pair arithmetic FP32; integration, wrap, reductions FP64",
             "device_ms_per_step": round(gpu_ms / args.steps, 5),
