@@ -46,6 +46,14 @@ def density_bytes_per_launch(n, n_pairs, nd, ncell):
     return n * (16 + 4 + 4 * nd + 4) + 4.0 * n_pairs + 4.0 * (ncell + 1)
 
 
+def skin_density_bytes_per_launch(n, n_pairs, nd, n_skin):
+    """Algorithmic HBM bytes of one density pass of skin mode: per bead its
+    position (16) and skin-list count (4) read, densities (4 nd) and pair
+    count (4) written; 4 bytes per skin-list entry read and per pair-list
+    entry written."""
+    return n * (16 + 4 + 4 * nd + 4) + 4.0 * n_skin + 4.0 * n_pairs
+
+
 def force_bytes_per_launch(n, n_pairs, nd):
     """Algorithmic HBM bytes of one force pass: per bead its own sorted
     position (16) and velocity (16), densities (4 nd), pair count (4), slot
@@ -127,7 +135,7 @@ class Workload:
                          max(int(math.floor(q.box[2])), 1) for q in plist)
         self.nd = plist[0].n_dens
 
-    def start(self, Engine, e=None, warmup=1):
+    def start(self, Engine, e=None, warmup=1, skin=0.0):
         """Uploads the initial state (into `e`, or a new engine) and runs
         the warm-up steps."""
         from implicit_dpd_amd import configs
@@ -136,6 +144,8 @@ class Workload:
                  else Engine(self.params, capacity=self.n))
             if self.ensemble:
                 e.set_tuning(nlist_width=256)  # dense corners of the sampled range
+            if skin > 0.0:
+                e.set_tuning(verlet_skin=skin, verlet_width=512 if self.ensemble else 256)
         st = configs.concat_systems(self.systems) if self.ensemble else self.systems
         e.upload(st["pos"], st["vel"], st["ids"], st["prop"], st["amp_id"], st["unit"])
         if self.bonded:
@@ -182,6 +192,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--chains", type=int, default=262144)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="neighbour-list skin (0: cell list rebuilt every step)")
     ap.add_argument("--calib-bytes", type=float, default=0.0,
                     help="only run the copy-kernel probe of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -204,7 +216,7 @@ def main():
     rho_mean = n / w.volume
     log(f"[bench] device {device_name()}; workload {w.name}: {n} beads")
     t = time.perf_counter()
-    e = w.start(Engine, warmup=args.warmup)
+    e = w.start(Engine, warmup=args.warmup, skin=args.skin)
     log(f"[bench] warmup {args.warmup} steps: {time.perf_counter() - t:.2f} s")
     if args.autotune:
         tt = e.autotune(10)
@@ -221,9 +233,12 @@ def main():
     # idle on entry and synchronised on exit; HIP events on the engine stream
     # give the device time of the same region
     e.sync()
+    b0 = e.skin_builds()
     t = time.perf_counter()
     gpu_ms = e.run_timed(args.steps)
     wall = time.perf_counter() - t
+    builds, n_skin = e.skin_stats()
+    builds -= b0
     ms_step = wall * 1e3 / args.steps
     value = n / (ms_step * 1e-3)
     log(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step wall, "
@@ -248,8 +263,10 @@ def main():
 
     prof = profiled_counters(WORKLOAD if w.name == "membrane" else f"{w.name}")
     kern = {}
+    dens_algo = (skin_density_bytes_per_launch(n, n_pairs, nd, n_skin) if args.skin > 0
+                 else density_bytes_per_launch(n, n_pairs, nd, w.ncell))
     for name, phase, algo in (
-            ("k_density", "density", density_bytes_per_launch(n, n_pairs, nd, w.ncell)),
+            ("k_density", "density", dens_algo),
             ("k_force", "force", force_bytes_per_launch(n, n_pairs, nd))):
         ms = ph[phase]
         gbs = algo / (ms * 1e-3) / 1e9
@@ -310,6 +327,10 @@ def main():
             "precision": "pair arithmetic FP32; integration, wrap, reductions FP64",
             "device_ms_per_step": round(gpu_ms / args.steps, 5),
             "phase_ms_eager": {k: round(v, 5) for k, v in ph.items()},
+            "neighbours": ({"mode": "skin list", "skin": args.skin,
+                            "builds_in_timed_steps": builds,
+                            "skin_entries_per_bead": round(n_skin / n, 2)}
+                           if args.skin > 0 else {"mode": "cell list rebuilt every step"}),
         },
         "roofline": roof,
         "step_hbm_roofline": step_hbm,

@@ -99,6 +99,8 @@ class MdpdTuning(C.Structure):
         ("nlist_width", C.c_int32),
         ("reorder_every", C.c_int32),
         ("use_graph", C.c_int32),
+        ("verlet_skin", C.c_float),
+        ("verlet_width", C.c_int32),
     ]
 
 
@@ -175,6 +177,7 @@ PROTOTYPES = {
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
     "mdpd_run_timed": (C.c_int, [VP, C.c_uint32, _DP]),
+    "mdpd_skin_stats": (C.c_int, [VP, P(C.c_uint64), P(C.c_uint64)]),
     # equation-of-state surrogate
     "mdpd_mlp_num_params": (C.c_int, [_U32P]),
     "mdpd_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, P(VP)]),

@@ -24,6 +24,7 @@ struct mdpd_ctx {
   std::vector<uint32_t> rep_off, cell_off, amp_off, amp_num_rep, sol_num_rep;
   int n_rep = 1;
   uint32_t cap = 0, n = 0, ncell = 0, amp_cap = 0, n_amp = 0;
+  uint32_t ncell_alloc = 0;  // cells the cell arrays hold (the cutoff grid)
   int ns = 1, nd = 1, env = 0;
   bool any_conv = false;
   uint32_t cm_corr = 100;
@@ -36,6 +37,9 @@ struct mdpd_ctx {
   size_t pinned_bytes = 0;
   uint32_t time = 0;
   bool drifted = false, have_cells = false, cells_fresh = false, have_density = false;
+  // skin mode: the skin list, its cells and the displacement reference match
+  // the bead storage (cleared by every call that moves or reorders beads)
+  bool skin_valid = false;
   // captured steps by (storage parity, virial, cm-drift step)
   hipGraphExec_t gexec[2][4] = {{nullptr, nullptr, nullptr, nullptr},
                                 {nullptr, nullptr, nullptr, nullptr}};

@@ -85,16 +85,26 @@ int check_params(const mdpd_params& p, std::string& why) {
   return MDPD_OK;
 }
 
-Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off, uint32_t nb) {
+// cell grid of a replica: the most cells per edge that are at least
+// min_cell wide (the cutoff, or 1 + skin in skin mode)
+void set_cells(Replica& r, double min_cell) {
+  for (int a = 0; a < 3; ++a) {
+    r.nc[a] = (int)std::floor(r.box[a] / min_cell);
+    r.h[a] = r.box[a] / r.nc[a];
+    r.inv_h[a] = 1.0 / r.h[a];
+    r.hf[a] = (float)r.h[a];
+  }
+  r.n_cells = (uint32_t)r.nc[0] * r.nc[1] * r.nc[2];
+}
+
+Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off, uint32_t nb,
+                     double min_cell) {
   Replica r{};
   for (int a = 0; a < 3; ++a) {
     r.box[a] = p.box[a];
     r.inv_box[a] = 1.0 / p.box[a];
-    r.nc[a] = (int)std::floor(p.box[a] / 1.0);
-    r.h[a] = p.box[a] / r.nc[a];
-    r.inv_h[a] = 1.0 / r.h[a];
-    r.hf[a] = (float)r.h[a];
   }
+  set_cells(r, min_cell);
   r.dt = p.dt;
   r.rd2 = p.rd * p.rd;
   r.conv_p = p.conv_p;
@@ -103,7 +113,6 @@ Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off,
   r.cell_off = cell_off;
   r.bead_off = bead_off;
   r.n_beads = nb;
-  r.n_cells = (uint32_t)r.nc[0] * r.nc[1] * r.nc[2];
   r.pair_seed = p.pair_seed;
   r.chem_seed = p.chem_seed;
   r.conv_from = p.conv_from;
@@ -167,7 +176,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
     c->rep_off[r] = tot;
     c->cell_off[r] = cells;
     const uint32_t nbr = nb ? nb[r] : capacity;
-    Replica rr = make_replica(p[r], cells, tot, nbr);
+    Replica rr = make_replica(p[r], cells, tot, nbr, 1.0);
     c->R.push_back(rr);
     tot += nbr;
     cells += rr.n_cells;
@@ -180,6 +189,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
     return fail(nullptr, MDPD_ERR_ARG, "capacity must be in 1.." + std::to_string(kIndexMask));
   }
   c->ncell = cells;
+  c->ncell_alloc = cells;
   c->amp_cap = c->cap / 4 + 1;
   c->amp_off.assign(n_rep + 1, 0);
   c->amp_num_rep.assign(n_rep, 0);
@@ -190,6 +200,8 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   c->tune.nlist_width = 96;
   c->tune.reorder_every = 0;
   c->tune.use_graph = 1;
+  c->tune.verlet_skin = 0.0f;
+  c->tune.verlet_width = 256;
 
   DevState& d = c->d;
   const uint32_t cap = c->cap;
@@ -245,6 +257,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.amp_off, n_rep + 1);
   ALLOC(d.d_time, 1);
   ALLOC(d.flags, 1);
+  ALLOC(d.rebuild, 2);
   ALLOC(d.red, (size_t)n_rep * 64 * 32 + 3 * (size_t)cap);
   ALLOC(d.red_out, (size_t)n_rep * 16);
   ALLOC(c->vir_bonded, (size_t)n_rep * 3);
@@ -292,6 +305,35 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
     if (prof) hipEventRecord(ev[k], st);
   };
   mark(0);
+  if (c->tune.verlet_skin > 0.0f) {
+    // skin mode: the drift raises the rebuild word; the cell list, the
+    // storage permutation and the skin list run only when it is set
+    launch_drift_skin(d, st);
+    mark(1);
+    launch_keys_gated(d, st);
+    launch_scan_cells_gated(d, st);
+    launch_scatter_gated(d, st);
+    launch_cell_sort_gated(d, st);
+    mark(2);
+    launch_permute_gated(d, st);
+    launch_copy_back_gated(d, st);
+    launch_skin_build(d, st);
+    mark(3);
+    launch_density_skin(d, c->nd, c->tune.block_scan, st);
+    mark(4);
+    launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true);
+    mark(5);
+    if (virial) launch_bead_vir_reduce(d, c->vir_bonded, st);
+    mark(6);
+    mark(7);
+    if (cm) launch_cm_drift(d, st);
+    mark(8);
+    if (c->any_conv) launch_head_conversion(d, st);
+    mark(9);
+    launch_step_end(d, st);
+    mark(10);
+    return;
+  }
   // the cell histogram is zero here: allocated zeroed and cleared by every scan
   launch_drift(d, true, false, st);
   mark(1);
@@ -339,11 +381,16 @@ int mdpd::check_flags(mdpd_ctx* c) {
 }
 
 int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
+  const bool skin = c->tune.verlet_skin > 0.0f;
+  // skin mode: the first step builds when the host has moved or reordered
+  // beads since the last build
+  if (skin && nsteps && !c->skin_valid) launch_fill_u32(c->d.rebuild, 1, 1u, c->st);
   for (uint32_t k = 0; k < nsteps; ++k) {
     const bool virial = (k + 1 == nsteps);
     const bool cm = (c->time % c->cm_corr) == 0;
     if (c->tune.use_graph && !c->profiling) {
       const int v = (virial ? 1 : 0) | (cm ? 2 : 0);
+      // (a skin-mode step keeps the storage where it is: no parity flip)
       const int p = c->parity;
       if (!c->gexec[p][v]) {
         // capturing records the step and flips the host's buffer parity, as
@@ -354,7 +401,7 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
         HIPCHK(c, hipStreamEndCapture(c->st, &g));
         HIPCHK(c, hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0));
         hipGraphDestroy(g);
-      } else {
+      } else if (!skin) {
         swap_master(c);
       }
       HIPCHK(c, hipGraphLaunch(c->gexec[p][v], c->st));
@@ -367,6 +414,11 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   c->have_cells = true;
   c->cells_fresh = true;
   c->have_density = true;
+  // beads sit up to skin / 2 away from where the last build put them
+  if (skin && nsteps) {
+    c->d.cell_slack = c->tune.verlet_skin;
+    c->skin_valid = true;
+  }
   return MDPD_OK;
 }
 
@@ -419,6 +471,7 @@ int mdpd_num_beads(const mdpd_ctx* c, uint32_t* n) {
 int mdpd_upload_soa(mdpd_ctx* c, uint32_t n, const double* pos, const double* vel,
                     const double* acc, const uint32_t* id, const uint32_t* prop,
                     const uint32_t* amp_id, const uint32_t* unit) {
+  if (c) c->skin_valid = false;
   if (!c || !pos || !vel || !id || !prop) return fail(c, MDPD_ERR_ARG, "null argument");
   if (n > c->cap) return fail(c, MDPD_ERR_CAPACITY, "more beads than the capacity");
   if (c->n_rep > 1 && n != c->cap)
@@ -640,6 +693,7 @@ int mdpd_download_fpdpd(mdpd_ctx* c, void* aos, uint32_t n, size_t stride) {
 
 int mdpd_set_topology(mdpd_ctx* c, const uint32_t* topo, uint32_t n_amp, const uint32_t* core_amp_id,
                       uint32_t n_core) {
+  if (c) c->skin_valid = false;
   if (!c || (n_amp && !topo)) return fail(c, MDPD_ERR_ARG, "null argument");
   if (n_amp > c->amp_cap) return fail(c, MDPD_ERR_CAPACITY, "too many amphiphiles");
   std::vector<double> cfb(n_amp);
@@ -706,12 +760,25 @@ int mdpd_get_clock(const mdpd_ctx* c, uint32_t* time) {
 }
 
 int mdpd_set_tuning(mdpd_ctx* c, const mdpd_tuning* t) {
+  if (c) c->skin_valid = false;
   if (!c || !t) return MDPD_ERR_ARG;
   auto okb = [](int b) { return b == 64 || b == 128 || b == 256; };
   if (!okb(t->block_scan) || !okb(t->block_force)) return fail(c, MDPD_ERR_ARG, "block sizes must be 64, 128 or 256");
   if (t->cell_div != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "cell_div must be 1");
   if (t->nlist_width < 8 || t->nlist_width > 1024 || (t->nlist_width & 3))
     return fail(c, MDPD_ERR_ARG, "nlist_width must be a multiple of 4 in 8..1024");
+  const float sk = t->verlet_skin;
+  const bool skin = sk > 0.0f;
+  if (!(sk == 0.0f || (sk >= 0.05f && sk <= 1.0f)))
+    return fail(c, MDPD_ERR_ARG, "verlet_skin must be 0 or in 0.05..1");
+  if (skin && (t->verlet_width < 8 || t->verlet_width > 2048 || (t->verlet_width & 3)))
+    return fail(c, MDPD_ERR_ARG, "verlet_width must be a multiple of 4 in 8..2048");
+  if (skin)
+    for (int r = 0; r < c->n_rep; ++r)
+      for (int a = 0; a < 3; ++a)
+        if (std::floor(c->R[r].box[a] / (1.0 + sk)) < 3.0)
+          return fail(c, MDPD_ERR_ARG, "replica " + std::to_string(r) +
+                                           ": a box edge holds fewer than 3 cells of 1 + verlet_skin");
   if (t->nlist_width != c->tune.nlist_width) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     const uint32_t cap64 = div_up(c->cap, 64) * 64;
@@ -726,6 +793,54 @@ int mdpd_set_tuning(mdpd_ctx* c, const mdpd_tuning* t) {
     c->d.W = t->nlist_width;
     c->have_density = false;
   }
+  DevState& d = c->d;
+  if (sk != c->tune.verlet_skin) {
+    // new cell grid (cells only grow with the skin, so the cell arrays
+    // allocated for the cutoff grid hold it)
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    uint32_t cells = 0;
+    for (int r = 0; r < c->n_rep; ++r) {
+      set_cells(c->R[r], skin ? 1.0 + (double)sk : 1.0);
+      c->R[r].cell_off = cells;
+      c->cell_off[r] = cells;
+      cells += c->R[r].n_cells;
+    }
+    c->cell_off[c->n_rep] = cells;
+    HIPCHK(c, hipMemcpyAsync(d.reps, c->R.data(), sizeof(Replica) * c->n_rep,
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.rep_cell_off, c->cell_off.data(), sizeof(uint32_t) * (c->n_rep + 1),
+                             hipMemcpyHostToDevice, c->st));
+    // the step's cell histogram starts from zero
+    launch_fill_u32(d.cell_cnt, c->ncell_alloc + 1, 0u, c->st);
+    c->ncell = cells;
+    d.ncell = cells;
+    const double rv = 1.0 + (double)sk + 1e-3;
+    d.vcut2 = (float)(rv * rv);
+    d.disp2 = 0.25 * (double)sk * (double)sk;
+    d.cell_slack = 0.0;
+    c->have_cells = c->cells_fresh = c->have_density = false;
+  }
+  if (skin && (!d.vlist || t->verlet_width != d.Wv)) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const uint32_t cap64 = div_up(c->cap, 64) * 64;
+    uint32_t* vl = nullptr;
+    HIPCHK(c, hipMalloc(&vl, sizeof(uint32_t) * (size_t)cap64 * t->verlet_width));
+    auto it = d.vlist ? std::find(c->allocs.begin(), c->allocs.end(), (void*)d.vlist)
+                      : c->allocs.end();
+    if (it != c->allocs.end()) {
+      hipFree(*it);
+      *it = vl;
+    } else {
+      c->allocs.push_back(vl);
+    }
+    d.vlist = vl;
+    d.Wv = t->verlet_width;
+    if (!d.vcount) {
+      int rc;
+      if ((rc = dalloc(c, &d.vcount, cap64)) != MDPD_OK) return rc;
+      if ((rc = dalloc(c, &d.dref, 3 * (size_t)c->cap)) != MDPD_OK) return rc;
+    }
+  }
   c->tune = *t;
   invalidate_graphs(c);
   return MDPD_OK;
@@ -738,6 +853,7 @@ int mdpd_get_tuning(const mdpd_ctx* c, mdpd_tuning* t) {
 }
 
 int mdpd_drift_predict(mdpd_ctx* c) {
+  if (c) c->skin_valid = false;
   if (!c) return MDPD_ERR_ARG;
   launch_drift(c->d, false, false, c->st);
   c->drifted = true;
@@ -746,6 +862,7 @@ int mdpd_drift_predict(mdpd_ctx* c) {
 }
 
 int mdpd_rebuild_cells(mdpd_ctx* c) {
+  if (c) c->skin_valid = false;
   if (!c) return MDPD_ERR_ARG;
   const DevState& d = c->d;
   launch_fill_u32(d.cell_cnt, d.ncell + 1, 0u, c->st);
@@ -758,6 +875,7 @@ int mdpd_rebuild_cells(mdpd_ctx* c) {
   launch_permute(d, c->drifted, true, c->st);
   swap_master(c);
   launch_topo_remap(c->d, c->st);
+  c->d.cell_slack = 0.0;
   c->have_cells = c->cells_fresh = true;
   c->have_density = false;
   return MDPD_OK;
@@ -849,6 +967,23 @@ int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
   hipEventDestroy(e1);
   if (rc) return rc;
   return check_flags(c);
+}
+
+int mdpd_skin_stats(mdpd_ctx* c, uint64_t* builds, uint64_t* entries) {
+  if (!c || !builds) return MDPD_ERR_ARG;
+  uint32_t v = 0;
+  HIPCHK(c, hipMemcpyAsync(&v, c->d.rebuild + 1, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *builds = v;
+  if (entries) {
+    *entries = 0;
+    if (c->d.vcount && c->d.n) {
+      std::vector<uint32_t> cnt(c->d.n);
+      HIPCHK(c, hipMemcpy(cnt.data(), c->d.vcount, 4 * (size_t)c->d.n, hipMemcpyDeviceToHost));
+      for (uint32_t k : cnt) *entries += std::min<uint32_t>(k, (uint32_t)c->d.Wv);
+    }
+  }
+  return MDPD_OK;
 }
 
 int mdpd_sync(mdpd_ctx* c) {

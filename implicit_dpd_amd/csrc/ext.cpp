@@ -34,6 +34,7 @@ extern "C" {
 // cone test and insertion run on the device, the per-core geometry (a few
 // vectors) on the host.
 int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
+  if (c) c->skin_valid = false;
   if (!c || !cp || !stop) return fail(c, MDPD_ERR_ARG, "null argument");
   if (c->n_rep != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
   if (!c->cells_fresh) return fail(c, MDPD_ERR_STATE, "chemistry needs the cell list of the current positions");
@@ -159,8 +160,8 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "autotune between drift and kick");
   DevState& d = c->d;
   char keybuf[256];
-  snprintf(keybuf, sizeof(keybuf), "n=%u,ncell=%u,nrep=%d,ns=%d,nd=%d,env=%d,namp=%u,W=%d", d.n,
-           d.ncell, c->n_rep, c->ns, c->nd, c->env, d.n_amp, d.W);
+  snprintf(keybuf, sizeof(keybuf), "n=%u,ncell=%u,nrep=%d,ns=%d,nd=%d,env=%d,namp=%u,W=%d,skin=%g",
+           d.n, d.ncell, c->n_rep, c->ns, c->nd, c->env, d.n_amp, d.W, (double)c->tune.verlet_skin);
   const std::string key(keybuf);
   const char* cache = std::getenv("MDPD_TUNE_CACHE");
   if (cache) {
@@ -222,6 +223,7 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
     }
     // the cell list and pair list of the trial steps are stale now
     c->cells_fresh = false;
+    c->skin_valid = false;
     c->have_density = false;
     c->time = t0;
     HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));

@@ -55,8 +55,14 @@ __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s)
          (s & 63u);
 }
 
-template <int ND, bool MULTI>
+// BUILD: the skin-list build of skin mode instead (rebuild steps only):
+// every bead but the bead itself with FP32 r^2 < vcut2 goes to the skin
+// list; vcut2 carries a margin over (1 + skin)^2 that absorbs the FP32
+// error, and cells are at least 1 + skin wide, so no pair that can come
+// inside the cutoff before the next rebuild is left out.
+template <int ND, bool MULTI, bool BUILD>
 __global__ void __launch_bounds__(256) k_density(DevState d) {
+  if (BUILD && *d.rebuild == 0u) return;
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -66,14 +72,15 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   const uint32_t cell_off = R.cell_off;
   const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
   const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
-  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
+  const float cut_hi = BUILD ? d.vcut2 : fmaxf(1.0f, rd2f) + kGuard;
   const float4 pi = d.spos[s];
   const uint32_t c = d.scell[s] - cell_off;
   const int cx = (int)(c % (uint32_t)nx);
   const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
   const int cz = (int)(c / (uint32_t)(nx * ny));
-  const uint32_t W = (uint32_t)d.W;
-  uint4* lst = const_cast<uint4*>(tile_list(d, s));
+  const uint32_t W = BUILD ? (uint32_t)d.Wv : (uint32_t)d.W;
+  uint4* lst = BUILD ? reinterpret_cast<uint4*>(d.vlist) + (size_t)(s >> 6) * (W >> 2) * 64u + (s & 63u)
+                     : const_cast<uint4*>(tile_list(d, s));
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
@@ -120,6 +127,19 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
+          if (BUILD) {
+            if (r2 >= cut_hi || t == s) continue;
+            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
+            if ((cnt & 3u) == 3u) {
+              if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
+            } else {
+              e0 = e1;
+              e1 = e2;
+              e2 = e;
+            }
+            ++cnt;
+            continue;
+          }
           if (r2 >= cut_hi) continue;
           bool in_rc = r2 < 1.0f, in_rd = r2 < rd2f;
           // next to a cutoff, or (near-)coincident: decide in FP64; the
@@ -164,15 +184,123 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
                                 : make_uint4(e0, e1, e2, 0u);
     lst[(size_t)(cnt >> 2) * 64u] = ch;
   }
+  if (BUILD) {
+    d.vcount[s] = cnt;
+  } else {
+#pragma unroll
+    for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+    d.ncount[s] = cnt;
+  }
+  if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
+}
+
+template <int ND, bool MULTI>
+static void dens_launch(const DevState& d, int block, hipStream_t st) {
+  hipLaunchKernelGGL((k_density<ND, MULTI, false>), dim3(div_up(d.n, block)), dim3(block), 0, st,
+                     d);
+}
+
+void launch_skin_build(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  const int block = 128;
+  if (d.n_rep > 1)
+    hipLaunchKernelGGL((k_density<1, true, true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+  else
+    hipLaunchKernelGGL((k_density<1, false, true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+}
+
+// Density pass of skin mode: the same decisions and sums as k_density, over
+// the skin list instead of the 27 cells (positions relative to the cells of
+// the last build, with the list's stencil codes), writing the same pair
+// list for the force pass.
+template <int ND, bool MULTI>
+__global__ void __launch_bounds__(256) k_density_skin(DevState d) {
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
+  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
+  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
+  const float4 pi = d.spos[s];
+  const uint32_t W = (uint32_t)d.W, Wv = (uint32_t)d.Wv;
+  uint4* lst = const_cast<uint4*>(tile_list(d, s));
+  const uint4* vl = reinterpret_cast<const uint4*>(d.vlist) + (size_t)(s >> 6) * (Wv >> 2) * 64u +
+                    (s & 63u);
+  uint32_t vc = d.vcount[s];
+  vc = vc < Wv ? vc : Wv;
+  float rho[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) rho[k] = 0.f;
+  uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
+  auto cand = [&](uint32_t e) {
+    const uint32_t t = e & kIndexMask;
+    const uint32_t code = e >> kCodeShift;
+    const float4 pj = d.spos[t];
+    const float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+    const float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
+    const float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 >= cut_hi) return;
+    bool in_rc = r2 < 1.0f, in_rd = r2 < rd2f;
+    if (fabsf(r2 - 1.0f) <= kGuard || fabsf(r2 - rd2f) <= kGuard || r2 < 1e-6f) {
+      const double r2d = fp64_r2(d, R, s, t);
+      in_rc = r2d < 1.0;
+      in_rd = r2d < R.rd2 && r2d != 0.0;
+    }
+    if (in_rd) {
+      const float rr = __builtin_amdgcn_sqrtf(r2);
+      const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
+      const float v = crho * w * w;
+      if (ND == 1) rho[0] += v;
+      else {
+        const uint32_t tj = __float_as_uint(pj.w) & 3u;
+#pragma unroll
+        for (int q = 0; q < ND; ++q) rho[q] += (tj == (uint32_t)q) ? v : 0.f;
+      }
+    }
+    if (in_rc) {
+      if ((cnt & 3u) == 3u) {
+        if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
+      } else {
+        e0 = e1;
+        e1 = e2;
+        e2 = e;
+      }
+      ++cnt;
+    }
+  };
+  for (uint32_t k = 0; k < vc; k += 4) {
+    const uint4 ch = vl[(size_t)(k >> 2) * 64u];
+    cand(ch.x);
+    if (k + 1 < vc) cand(ch.y);
+    if (k + 2 < vc) cand(ch.z);
+    if (k + 3 < vc) cand(ch.w);
+  }
+  const uint32_t tail = cnt & 3u;
+  if (tail != 0u && cnt < W) {
+    const uint4 ch = tail == 1u ? make_uint4(e2, 0u, 0u, 0u)
+                   : tail == 2u ? make_uint4(e1, e2, 0u, 0u)
+                                : make_uint4(e0, e1, e2, 0u);
+    lst[(size_t)(cnt >> 2) * 64u] = ch;
+  }
 #pragma unroll
   for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
   d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
 }
 
-template <int ND, bool MULTI>
-static void dens_launch(const DevState& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL((k_density<ND, MULTI>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st) {
+  if (!d.n) return;
+  const dim3 g(div_up(d.n, block)), b(block);
+  const bool multi = d.n_rep > 1;
+  if (nd == 1) {
+    if (multi) hipLaunchKernelGGL((k_density_skin<1, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_density_skin<1, false>), g, b, 0, st, d);
+  } else {
+    if (multi) hipLaunchKernelGGL((k_density_skin<3, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_density_skin<3, false>), g, b, 0, st, d);
+  }
 }
 
 void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_t st) {

@@ -119,7 +119,8 @@ void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t s
                      predict_only ? 1 : 0);
 }
 
-__global__ void __launch_bounds__(256) k_keys(DevState d) {
+__global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
+  if (gate && *d.rebuild == 0u) return;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i0 < d.n;
   uint32_t k = 0;
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(256) k_keys(DevState d) {
 }
 void launch_keys(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d, 0);
 }
 
 // exclusive scan of the cell histogram (1024 cells per block, three passes)
@@ -169,7 +170,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& excl) 
 // The histogram is cleared as it is read, ready for the next rebuild.
 __global__ void __launch_bounds__(256) k_scan1(uint32_t* __restrict__ in,
                                                uint32_t* __restrict__ out,
-                                               uint32_t* __restrict__ bsum, uint32_t m) {
+                                               uint32_t* __restrict__ bsum, uint32_t m,
+                                               const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
   uint32_t v[4], t = 0;
 #pragma unroll
@@ -215,7 +218,9 @@ __global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint
 __global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
                                                uint32_t* __restrict__ cursor,
                                                const uint32_t* __restrict__ bsum, uint32_t m,
-                                               uint32_t total, int carry) {
+                                               uint32_t total, int carry,
+                                               const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   const uint32_t base = blockIdx.x * 1024u;
   uint32_t add;
   if (carry) {
@@ -241,17 +246,20 @@ __global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
   if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total;
 }
 
-void launch_scan_cells(const DevState& d, hipStream_t st) {
+static void scan_cells(const DevState& d, const uint32_t* gate, hipStream_t st) {
   const uint32_t m = d.ncell;
   const uint32_t nb = div_up(m, 1024u);
-  hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, st, d.cell_cnt, d.cell_start, d.bsum, m);
+  hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, st, d.cell_cnt, d.cell_start, d.bsum, m,
+                     gate);
   const int carry = nb <= 4096u ? 1 : 0;  // up to 4M cells: no separate block-sum pass
   if (!carry) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb);
   hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, st, d.cell_start, d.cursor, d.bsum, m, d.n,
-                     carry);
+                     carry, gate);
 }
+void launch_scan_cells(const DevState& d, hipStream_t st) { scan_cells(d, nullptr, st); }
 
-__global__ void __launch_bounds__(256) k_scatter(DevState d) {
+__global__ void __launch_bounds__(256) k_scatter(DevState d, int gate) {
+  if (gate && *d.rebuild == 0u) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < d.n;
   const uint32_t k = active ? d.key[i] : 0u;
@@ -260,12 +268,13 @@ __global__ void __launch_bounds__(256) k_scatter(DevState d) {
 }
 void launch_scatter(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d, 0);
 }
 
 // ascending bead slots inside each cell, so the sorted order (and every FP32
 // sum over it) does not depend on the order of the scatter atomics
-__global__ void __launch_bounds__(256) k_cell_sort(DevState d) {
+__global__ void __launch_bounds__(256) k_cell_sort(DevState d, int gate) {
+  if (gate && *d.rebuild == 0u) return;
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.ncell) return;
   const uint32_t b = d.cell_start[c], e = d.cell_start[c + 1];
@@ -282,14 +291,15 @@ __global__ void __launch_bounds__(256) k_cell_sort(DevState d) {
   }
 }
 void launch_cell_sort(const DevState& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d, 0);
 }
 
 // Moves every bead to its cell-order slot s (from old slot perm[s]) in the
 // other master buffer set, and writes the FP32 copies the pair passes read:
 // position inside the bead's cell (FP64 difference, then rounded) with
 // id << 2 | type in .w, and the velocity the force uses.
-__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full) {
+__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin) {
+  if (skin && *d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const uint32_t i = d.perm[s];
@@ -310,7 +320,10 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     const double vb = d.vbuf[a * cap + i];
     d.alt.pos[a * cap + s] = x;
     d.alt.vbuf[a * cap + s] = vb;
-    d.alt.delta[a * cap + s] = d.delta[a * cap + i];
+    const double dl = d.delta[a * cap + i];
+    d.alt.delta[a * cap + s] = dl;
+    // skin mode: the displacement test of the coming steps starts here
+    if (skin) d.dref[a * cap + s] = dl;
     l[a] = (float)(x - (double)cc[a] * R.h[a]);
     if (full) {
       d.alt.vel[a * cap + s] = d.vel[a * cap + i];
@@ -335,7 +348,7 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
-                     full ? 1 : 0);
+                     full ? 1 : 0, 0);
 }
 
 // chain bead slots follow the beads to their new slots
@@ -618,7 +631,13 @@ __global__ void __launch_bounds__(256) k_head_conv(DevState d) {
   const Replica& R = d.reps[r];
   if (R.conv_p <= 0.0 || d.prop[i] != (uint32_t)R.conv_from) return;
   Saru s = saru_seed(d.amp_id[i], *d.d_time, R.chem_seed);
-  if (saru_uni_f64(s) <= R.conv_p) d.prop[i] = (uint32_t)R.conv_to;
+  if (saru_uni_f64(s) <= R.conv_p) {
+    d.prop[i] = (uint32_t)R.conv_to;
+    // the type also rides in the FP32 copy (read by the pair passes before
+    // the next rebuild in skin mode)
+    reinterpret_cast<float*>(d.spos + i)[3] =
+        __uint_as_float((d.id[i] << 2) | ((uint32_t)R.conv_to & 3u));
+  }
 }
 void launch_head_conversion(const DevState& d, hipStream_t st) {
   if (!d.n) return;
@@ -628,6 +647,118 @@ void launch_head_conversion(const DevState& d, hipStream_t st) {
 __global__ void k_time_inc(uint32_t* t) { *t += 1u; }
 void launch_time_inc(uint32_t* d_time, hipStream_t st) {
   hipLaunchKernelGGL(k_time_inc, dim3(1), dim3(1), 0, st, d_time);
+}
+
+// ---------------------------------------------------------------- skin mode
+// drift_and_predict (driftkick.hpp:5-30, the same FP64 expressions as
+// k_drift) in place, plus what the pair passes read between rebuilds: the
+// FP32 position relative to the cell of the last build (minimum image, so
+// a bead that crossed the box edge stays next to its cell), the predicted
+// velocity and the chain-major copy of chain beads.  A bead that has moved
+// more than skin / 2 since the last build raises the rebuild word.
+__global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
+#pragma clang fp contract(off)
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
+  const Replica& R = d.reps[r];
+  const double dt = R.dt;
+  const uint32_t cap = d.cap;
+  const uint32_t c = d.scell[i] - R.cell_off;
+  const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
+  const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
+  double p[3], disp = 0.0;
+  float l[3], v[3];
+  bool fin = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double vv = d.vel[a * cap + i];
+    const double ac = (double)d.acc[a * cap + i];
+    const double dr = dt * (vv + 0.5 * dt * ac);
+    const double tr = d.pos[a * cap + i] + dr;
+    const double dl = d.delta[a * cap + i] + dr;
+    d.delta[a * cap + i] = dl;
+    p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
+    d.pos[a * cap + i] = p[a];
+    const double vb = vv + 0.5 * dt * ac;
+    d.vbuf[a * cap + i] = vb;
+    fin = fin && isfinite(p[a]) && isfinite(vv);
+    const double e = dl - d.dref[a * cap + i];
+    disp += e * e;
+    double x = p[a] - (double)cc[a] * R.h[a];
+    if (x > 0.5 * R.box[a]) x -= R.box[a];
+    else if (x < -0.5 * R.box[a]) x += R.box[a];
+    l[a] = (float)x;
+    v[a] = (float)vb;
+  }
+  if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
+  if (!(disp <= d.disp2)) *d.rebuild = 1u;  // every writer stores the same value
+  float* sp = reinterpret_cast<float*>(d.spos + i);  // .w (id, type) stays
+  sp[0] = l[0];
+  sp[1] = l[1];
+  sp[2] = l[2];
+  d.svel[i] = make_float4(v[0], v[1], v[2], 0.f);
+  const uint32_t ch = d.chain[i];
+  if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(p[0], p[1], p[2], 0.0);
+}
+void launch_drift_skin(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_drift_skin, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+void launch_keys_gated(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d, 1);
+}
+void launch_scan_cells_gated(const DevState& d, hipStream_t st) { scan_cells(d, d.rebuild, st); }
+void launch_scatter_gated(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d, 1);
+}
+void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d, 1);
+}
+void launch_permute_gated(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1);
+}
+
+// The rebuild's permutation wrote the other buffer set; copy it back so the
+// master storage keeps one address whether or not this step rebuilt (the
+// captured step cannot know).  vel, acc and press are rewritten by the
+// force pass of the step and need no copy.
+__global__ void __launch_bounds__(256) k_copy_back(DevState d) {
+  if (*d.rebuild == 0u) return;
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const uint32_t cap = d.cap;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    d.pos[a * cap + s] = d.alt.pos[a * cap + s];
+    d.vbuf[a * cap + s] = d.alt.vbuf[a * cap + s];
+    d.delta[a * cap + s] = d.alt.delta[a * cap + s];
+  }
+  d.id[s] = d.alt.id[s];
+  d.prop[s] = d.alt.prop[s];
+  d.amp_id[s] = d.alt.amp_id[s];
+  d.unit[s] = d.alt.unit[s];
+  d.orig[s] = d.alt.orig[s];
+  d.chain[s] = d.alt.chain[s];
+}
+void launch_copy_back_gated(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_copy_back, grid1(d.n, 256), dim3(256), 0, st, d);
+}
+
+// end of a skin-mode step: clock, build count (rebuild[1]), and the
+// rebuild word cleared for the next step's displacement test
+__global__ void k_step_end(DevState d) {
+  *d.d_time += 1u;
+  if (d.rebuild[0]) d.rebuild[1] += 1u;
+  d.rebuild[0] = 0u;
+}
+void launch_step_end(const DevState& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, d);
 }
 
 // bead counts along one axis per replica
@@ -677,7 +808,8 @@ __global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, ui
   int rad[3];
   bool brute = false;
   for (int a = 0; a < 3; ++a) {
-    rad[a] = (int)ceil(1.2 * R.inv_h[a] - 1e-12);
+    // beads may sit up to cell_slack / 2 outside their cell (skin mode)
+    rad[a] = (int)ceil((1.2 + d.cell_slack) * R.inv_h[a] - 1e-12);
     if (2 * rad[a] + 1 > R.nc[a]) brute = true;
   }
   auto visit = [&](uint32_t j) {

@@ -44,6 +44,15 @@ struct DevState : Master {
   uint32_t* nlist;      // wave-tiled pair list [cap/64][W/4][64][4]
   uint32_t* ncount;     // [cap]
   int32_t W;
+  // skin mode: neighbour list of radius 1 + skin, rebuilt on demand
+  uint32_t* vlist;      // [cap/64][Wv/4][64][4], slot | stencil code << 26
+  uint32_t* vcount;     // [cap]
+  double* dref;         // [3][cap] delta_sumr at the last build
+  uint32_t* rebuild;    // [0]: 1 = rebuild cells and list in this step; [1]: builds so far
+  int32_t Wv;
+  float vcut2;          // squared list radius (1 + skin + margin)^2
+  double disp2;         // (skin / 2)^2: a larger displacement forces a rebuild
+  double cell_slack;    // how far a bead may sit outside its cell (0 or skin)
   // replicas
   Replica* reps;
   uint32_t* rep_off;    // [n_rep + 1] first bead of each replica
@@ -94,6 +103,18 @@ void launch_chainpos_fill(const DevState& d, hipStream_t st);
 void launch_cm_drift(const DevState& d, hipStream_t st);
 void launch_head_conversion(const DevState& d, hipStream_t st);
 void launch_time_inc(uint32_t* d_time, hipStream_t st);
+// skin mode (launches of the rebuild kernels return at once unless
+// *d.rebuild is set)
+void launch_drift_skin(const DevState& d, hipStream_t st);
+void launch_keys_gated(const DevState& d, hipStream_t st);
+void launch_scan_cells_gated(const DevState& d, hipStream_t st);
+void launch_scatter_gated(const DevState& d, hipStream_t st);
+void launch_cell_sort_gated(const DevState& d, hipStream_t st);
+void launch_permute_gated(const DevState& d, hipStream_t st);
+void launch_copy_back_gated(const DevState& d, hipStream_t st);
+void launch_skin_build(const DevState& d, hipStream_t st);
+void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st);
+void launch_step_end(const DevState& d, hipStream_t st);
 // observables: out[n_rep][13] FP64; inv_orig scratch [cap]
 void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, uint32_t* inv_orig,
                     double* out, hipStream_t st);

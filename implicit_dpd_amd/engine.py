@@ -190,6 +190,16 @@ class Engine:
         self._c(self.L.mdpd_run_timed(self.h, int(nsteps), C.byref(ms)))
         return ms.value
 
+    def skin_stats(self):
+        """(steps so far that rebuilt cells and skin list, skin-list entries
+        of the last build) in skin mode."""
+        n, m = C.c_uint64(), C.c_uint64()
+        self._c(self.L.mdpd_skin_stats(self.h, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
+    def skin_builds(self):
+        return self.skin_stats()[0]
+
     def sync(self):
         self._c(self.L.mdpd_sync(self.h))
 

@@ -128,7 +128,9 @@ typedef struct mdpd_observables {
   uint32_t pad;
 } mdpd_observables;
 
-/* Launch choices the autotuner picks; results do not depend on them. */
+/* Launch choices the autotuner picks.  Block sizes, nlist_width and
+   use_graph do not change results; verlet_skin changes only the FP32
+   rounding of pair terms (the pair sets are the same). */
 typedef struct mdpd_tuning {
   int32_t block_scan;       /* threads per block, density/pair-list pass */
   int32_t block_force;      /* threads per block, force pass */
@@ -137,6 +139,15 @@ typedef struct mdpd_tuning {
   int32_t reorder_every;    /* ignored: bead storage is put in cell order at
                                every cell-list rebuild */
   int32_t use_graph;        /* capture each step in a hipGraph */
+  float verlet_skin;        /* 0: mdpd_run rebuilds the cell list every step.
+                               In 0.05..1: cells are at least 1 + skin wide,
+                               each build keeps every pair closer than
+                               1 + skin, and mdpd_run rebuilds cells and list
+                               only in steps where some bead has moved more
+                               than skin / 2 since the last build (decided on
+                               the device, inside the captured step).  Every
+                               box edge must hold 3 such cells. */
+  int32_t verlet_width;     /* skin-list slots per bead, multiple of 4 (8..2048) */
 } mdpd_tuning;
 
 typedef struct mdpd_ctx mdpd_ctx;
@@ -208,6 +219,11 @@ int mdpd_run(mdpd_ctx* ctx, uint32_t nsteps);
 /* mdpd_run with HIP events recorded on the context's stream around the
    steps; *gpu_ms is the device time between them.  Used by the benchmark. */
 int mdpd_run_timed(mdpd_ctx* ctx, uint32_t nsteps, double* gpu_ms);
+/* Skin mode (mdpd_tuning.verlet_skin > 0): the number of steps so far that
+   rebuilt cells and skin list, and (entries may be NULL) the skin-list
+   entries of the last build summed over the beads.  No reference
+   counterpart. */
+int mdpd_skin_stats(mdpd_ctx* ctx, uint64_t* builds, uint64_t* entries);
 /* Blocks until the context's stream is idle. */
 int mdpd_sync(mdpd_ctx* ctx);
 
