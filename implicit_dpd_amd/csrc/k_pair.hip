@@ -119,10 +119,10 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
           else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
         }
-        float4 pn = ta < tb ? d.spos[ta] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (uint32_t t = ta; t < tb; ++t) {
-          const float4 pj = pn;
-          if (t + 1 < tb) pn = d.spos[t + 1];  // next candidate in flight
+          // (other waves hide the load latency; a software prefetch costs
+          // six register moves per candidate)
+          const float4 pj = d.spos[t];
           const bool g2 = t >= b2, g1 = t >= b1;
           const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
@@ -146,7 +146,11 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           // bead itself (t == s, r2 == 0) lands here and is dropped
           if (fabsf(r2 - 1.0f) <= kGuard || fabsf(r2 - rd2f) <= kGuard || r2 < 1e-6f) {
             if (t == s) continue;
-            const double r2d = fp64_r2(d, R, s, t);
+            // opaque copy: keeps the compiler from stepping an FP64 address
+            // along with t in every iteration for this rare path
+            uint32_t tt = t;
+            asm volatile("" : "+v"(tt));
+            const double r2d = fp64_r2(d, R, s, tt);
             in_rc = r2d < 1.0;
             in_rd = r2d < R.rd2 && r2d != 0.0;
           }
@@ -162,14 +166,15 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
             }
           }
           if (in_rc) {
+            // the register window always shifts; every fourth entry the
+            // window plus the entry that fell out go out as one chunk
             const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
-            if ((cnt & 3u) == 3u) {
-              if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
-            } else {
-              e0 = e1;
-              e1 = e2;
-              e2 = e;
-            }
+            const uint32_t out = e0;
+            e0 = e1;
+            e1 = e2;
+            e2 = e;
+            if ((cnt & 3u) == 3u && cnt < W)
+              lst[(size_t)(cnt >> 2) * 64u] = make_uint4(out, e0, e1, e2);
             ++cnt;
           }
         }
