@@ -328,14 +328,15 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
     mark(7);
     if (cm) launch_cm_drift(d, st);
     mark(8);
+    // (a separate pass here: it also rewrites the FP32 type the next
+    // step's pair passes read without a rebuild)
     if (c->any_conv) launch_head_conversion(d, st);
     mark(9);
-    launch_step_end(d, st);
     mark(10);
     return;
   }
   // the cell histogram is zero here: allocated zeroed and cleared by every scan
-  launch_drift(d, true, false, st);
+  launch_drift(d, true, false, st, true);
   mark(1);
   launch_scan_cells(d, st);
   launch_scatter(d, st);
@@ -349,16 +350,17 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
   mark(4);
   // the force pass ends with ForceBonded::CalcListedForce and the kick of
   // each bead (bead-centric: every bead evaluates its own chain)
-  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true);
+  // ... and, with any reactive replica, the head conversion of each bead
+  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true, c->any_conv);
   mark(5);
   if (virial) launch_bead_vir_reduce(d, c->vir_bonded, st);
   mark(6);
   mark(7);
+  // the CM drift removal only changes velocities, so the conversion
+  // (main.cpp order: after it) may run inside the force pass
   if (cm) launch_cm_drift(d, st);
   mark(8);
-  if (c->any_conv) launch_head_conversion(d, st);
   mark(9);
-  launch_time_inc(d.d_time, st);
   mark(10);
 }
 
@@ -385,6 +387,9 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   // skin mode: the first step builds when the host has moved or reordered
   // beads since the last build
   if (skin && nsteps && !c->skin_valid) launch_fill_u32(c->d.rebuild, 1, 1u, c->st);
+  // the device clock holds the previous step's time; each step's first
+  // kernel advances it
+  if (nsteps) launch_fill_u32(c->d.d_time, 1, c->time - 1u, c->st);
   for (uint32_t k = 0; k < nsteps; ++k) {
     const bool virial = (k + 1 == nsteps);
     const bool cm = (c->time % c->cm_corr) == 0;
@@ -940,7 +945,6 @@ int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
   if (!c) return MDPD_ERR_ARG;
   if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
-  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
   int rc = run_steps(c, nsteps);
   if (rc) return rc;
   return check_flags(c);
@@ -953,7 +957,6 @@ int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  HIPCHK(c, hipMemcpyAsync(c->d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipEventRecord(e0, c->st));
   int rc = run_steps(c, nsteps);
   if (rc == MDPD_OK) {

@@ -220,6 +220,12 @@ void launch_skin_build(const DevState& d, hipStream_t st) {
 // list for the force pass.
 template <int ND, bool MULTI>
 __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // every rebuild kernel of this step has run: count the build and clear
+    // the word for the next step's displacement test
+    if (d.rebuild[0]) d.rebuild[1] += 1u;
+    d.rebuild[0] = 0u;
+  }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -337,7 +343,8 @@ __device__ __forceinline__ void fill_pair_table(const Replica& R, float4* tab) {
 }
 
 template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
-__global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead) {
+__global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead,
+                                               int conv) {
   // single system: the coefficients come from a block-shared table indexed
   // by the type pair instead of per-lane selects
   __shared__ float4 ptab[18];
@@ -460,6 +467,13 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     // acceleration just summed (the same value the stand-alone pass reads)
     float af[3] = {fx, fy, fz};
     bonded_kick_bead(d, R, s, af, VIRIAL, true, vir_bead);
+    if (conv && d.unit[s] == 0u && d.prop[s] == (uint32_t)R.conv_from && R.conv_p > 0.0) {
+      // the step's head conversion for this bead (same draw as
+      // k_head_conv); only the master type changes, and the pair passes
+      // read the FP32 copy, which the next rebuild rewrites
+      Saru q = saru_seed(d.amp_id[s], *d.d_time, R.chem_seed);
+      if (saru_uni_f64(q) <= R.conv_p) d.prop[s] = (uint32_t)R.conv_to;
+    }
   } else {
     d.acc[s] = fx;
     d.acc[d.cap + s] = fy;
@@ -468,32 +482,34 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
 }
 
 template <int NS, int ND, int ENV>
-static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb) {
+static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb,
+                          int cv) {
   const dim3 g(div_up(d.n, block)), b(block);
   const bool multi = d.n_rep > 1;
   if (multi) {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb, cv);
   } else {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb, cv);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb, cv);
   }
 }
 
 template <int NS, int ND>
 static void force_launch2(const DevState& d, int env, bool virial, int block, hipStream_t st,
-                          double* vb) {
-  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st, vb);
-  else force_launch3<NS, ND, 1>(d, virial, block, st, vb);
+                          double* vb, int cv) {
+  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st, vb, cv);
+  else force_launch3<NS, ND, 1>(d, virial, block, st, vb, cv);
 }
 
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st, bool fuse) {
+                  hipStream_t st, bool fuse, bool conv) {
   if (!d.n) return;
   double* vb = fuse ? bead_vir_buffer(d) : nullptr;
-  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st, vb);
-  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st, vb);
-  else force_launch2<3, 1>(d, env, virial, block, st, vb);
+  const int cv = (fuse && conv) ? 1 : 0;
+  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st, vb, cv);
+  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st, vb, cv);
+  else force_launch2<3, 1>(d, env, virial, block, st, vb, cv);
 }
 
 // ---------------------------------------------------------------- pair listing

@@ -76,8 +76,11 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t* base, uint32_t key, bool a
 // velocity; the predicted velocity (vel_buf == vel of the reference after
 // the drift) goes to vbuf.  With `keys` the cell key and the cell
 // histogram of the rebuild are produced in the same pass.
-__global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only) {
+__global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only, int tick) {
 #pragma clang fp contract(off)
+  // tick: the step's clock advances here (no kernel of the step before
+  // this one reads it)
+  if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i0 < d.n;
   const uint32_t i = active ? i0 : d.n - 1;
@@ -113,10 +116,10 @@ __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict
   }
   if (keys) (void)agg_inc(d.cell_cnt, k, active);
 }
-void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st) {
+void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st, bool tick) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, keys ? 1 : 0,
-                     predict_only ? 1 : 0);
+                     predict_only ? 1 : 0, tick ? 1 : 0);
 }
 
 __global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
@@ -303,7 +306,7 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const uint32_t i = d.perm[s];
-  d.inv_perm[i] = s;
+  if (full) d.inv_perm[i] = s;  // for the explicit rebuild's topology remap
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const uint32_t key = d.key[i];
@@ -658,6 +661,7 @@ void launch_time_inc(uint32_t* d_time, hipStream_t st) {
 // more than skin / 2 since the last build raises the rebuild word.
 __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
 #pragma clang fp contract(off)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;  // the step's clock
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.n) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
@@ -748,17 +752,6 @@ __global__ void __launch_bounds__(256) k_copy_back(DevState d) {
 void launch_copy_back_gated(const DevState& d, hipStream_t st) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_copy_back, grid1(d.n, 256), dim3(256), 0, st, d);
-}
-
-// end of a skin-mode step: clock, build count (rebuild[1]), and the
-// rebuild word cleared for the next step's displacement test
-__global__ void k_step_end(DevState d) {
-  *d.d_time += 1u;
-  if (d.rebuild[0]) d.rebuild[1] += 1u;
-  d.rebuild[0] = 0u;
-}
-void launch_step_end(const DevState& d, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, d);
 }
 
 // bead counts along one axis per replica

@@ -75,7 +75,9 @@ struct DevState : Master {
 
 // launch wrappers (defined in the .hip files)
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
-void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st);
+// tick: advance the device clock (the step's first kernel)
+void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st,
+                  bool tick = false);
 void launch_keys(const DevState& d, hipStream_t st);
 void launch_scan_cells(const DevState& d, hipStream_t st);
 void launch_scatter(const DevState& d, hipStream_t st);
@@ -89,8 +91,9 @@ void launch_topo_from_chain(const DevState& d, hipStream_t st);
 void launch_density(const DevState& d, int ns, int nd, int block, hipStream_t st);
 // fuse: the force pass ends with the bead's bonded terms and the kick
 // (virial steps: per-bead chain virials for launch_bead_vir_reduce)
+// conv (with fuse): the head conversion of the bead ends the epilogue
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st, bool fuse = false);
+                  hipStream_t st, bool fuse = false, bool conv = false);
 double* bead_vir_buffer(const DevState& d);
 void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st);
 void launch_kick(const DevState& d, hipStream_t st);
@@ -113,8 +116,8 @@ void launch_cell_sort_gated(const DevState& d, hipStream_t st);
 void launch_permute_gated(const DevState& d, hipStream_t st);
 void launch_copy_back_gated(const DevState& d, hipStream_t st);
 void launch_skin_build(const DevState& d, hipStream_t st);
+// also counts the build of this step (rebuild[1]) and clears rebuild[0]
 void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st);
-void launch_step_end(const DevState& d, hipStream_t st);
 // observables: out[n_rep][13] FP64; inv_orig scratch [cap]
 void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, uint32_t* inv_orig,
                     double* out, hipStream_t st);

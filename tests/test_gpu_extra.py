@@ -149,6 +149,40 @@ def test_head_conversion_matches_oracle(engine_mod, oracle):
     assert abs(total - m) < 5 * sd, (total, m)
 
 
+@pytest.mark.parametrize("skin", [0.0, 0.3])
+def test_run_head_conversion_matches_oracle(engine_mod, oracle, skin):
+    """Head conversion inside mdpd_run (the force pass's epilogue, or its own
+    pass in skin mode): the types after six steps equal six keyed
+    conversions of the oracle at the same clock values (types do not depend
+    on the trajectory)."""
+    box = (12.0, 12.0, 12.0)
+    p = configs.warren_params(box, A=configs.MEMBRANE_A, B=25.0, rd=0.75, n_species=3, pair_seed=9)
+    p.conv_p, p.conv_from, p.conv_to = 0.05, configs.HYPHIL, configs.SOLVENT
+    p.chem_seed = 4242
+    s = configs.random_chains(4000, box, 8)
+    n = len(s["pos"])
+    e = engine_mod.Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(40, p.pair_seed)
+    if skin:
+        e.set_tuning(verlet_skin=skin)
+    e.run(4)
+    e.run(2)
+    g = e.download()
+    assert e.get_clock() == 46
+    e.close()
+    o = oracle.System(p, n, 4001)
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    o.set_topology(s["topo"])
+    for t in range(40, 46):
+        o.set_clock(t)
+        o.head_conversion()
+    r = o.get()
+    assert np.array_equal(g["prop"], r["prop"])
+    assert int((g["prop"] != s["prop"]).sum()) > 0
+
+
 def test_chem_event_matches_oracle(engine_mod, oracle):
     box = (8.0, 9.0, 8.0)
     n_amp = 500
