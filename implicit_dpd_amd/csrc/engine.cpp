@@ -336,14 +336,14 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
     return;
   }
   // the cell histogram is zero here: allocated zeroed and cleared by every scan
-  launch_drift(d, true, false, st, true);
+  launch_step_keys(d, st);
   mark(1);
   launch_scan_cells(d, st);
   launch_scatter(d, st);
   launch_cell_sort(d, st);
   mark(2);
   // bead storage moves to the new cell order; the slots below are the new ones
-  launch_permute(d, true, false, st);
+  launch_step_permute(d, st);
   swap_master(c);
   mark(3);
   launch_density(d, c->ns, c->nd, c->tune.block_scan, st);

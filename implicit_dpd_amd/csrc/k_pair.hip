@@ -194,6 +194,9 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   } else {
 #pragma unroll
     for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+    // one density: also in the velocity copy's spare lane, so the force
+    // pass gets a neighbour's velocity and density in one 16-byte load
+    if (ND == 1) reinterpret_cast<float*>(d.svel + s)[3] = rho[0];
     d.ncount[s] = cnt;
   }
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
@@ -297,6 +300,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
   }
 #pragma unroll
   for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+  if (ND == 1) reinterpret_cast<float*>(d.svel + s)[3] = rho[0];
   d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
 }
@@ -435,8 +439,12 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     }
     const float cons_c = has_c1 ? a0 + c1 * fmaxf(rr - arc, 0.0f) : a0;
     float mbs = 0.f;
+    if (ND == 1) {
+      mbs = mbk[0] * (rho_i[0] + vj.w);
+    } else {
 #pragma unroll
-    for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
+      for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
+    }
     const float wm = rdm1 ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
     const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
     const float fs = (cons_c * wc + mbs * wm + sg * wR * xi - gm * wD * rv * rinv) * rinv;

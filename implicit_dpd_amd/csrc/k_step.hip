@@ -76,8 +76,30 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t* base, uint32_t key, bool a
 // velocity; the predicted velocity (vel_buf == vel of the reference after
 // the drift) goes to vbuf.  With `keys` the cell key and the cell
 // histogram of the rebuild are produced in the same pass.
-__global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only, int tick) {
+// One axis of drift_and_predict (driftkick.hpp:5-30) in FP64 without
+// contraction: the wrapped position, delta_sumr and the predicted velocity.
+// Every kernel that drifts uses this, so they agree bit for bit.
+struct DriftAxis {
+  double p, dl, vb;
+};
+__device__ __forceinline__ DriftAxis drift_axis(const Replica& R, int a, double x, double v,
+                                                double ac, double dl) {
 #pragma clang fp contract(off)
+  const double dt = R.dt;
+  const double dr = dt * (v + 0.5 * dt * ac);
+  const double tr = x + dr;
+  DriftAxis o;
+  o.dl = dl + dr;
+  o.p = tr - floor(tr * R.inv_box[a]) * R.box[a];
+  o.vb = v + 0.5 * dt * ac;
+  return o;
+}
+
+// store = 0 (the step): only the cell key and histogram of the drifted
+// position; the permutation that follows drifts each bead again as it moves
+// it and writes the new state once.
+__global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only, int tick,
+                                               int store) {
   // tick: the step's clock advances here (no kernel of the step before
   // this one reads it)
   if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
@@ -88,7 +110,6 @@ __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict
   if (active) {
     const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
     const Replica& R = d.reps[r];
-    const double dt = R.dt;
     const uint32_t cap = d.cap;
     double p[3];
     bool fin = true;
@@ -96,16 +117,19 @@ __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict
     for (int a = 0; a < 3; ++a) {
       const double v = d.vel[a * cap + i];
       const double ac = (double)d.acc[a * cap + i];
+      const double x = d.pos[a * cap + i];
       if (!predict_only) {
-        const double dr = dt * (v + 0.5 * dt * ac);
-        const double tr = d.pos[a * cap + i] + dr;
-        d.delta[a * cap + i] += dr;
-        p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
-        d.pos[a * cap + i] = p[a];
+        const DriftAxis o = drift_axis(R, a, x, v, ac, store ? d.delta[a * cap + i] : 0.0);
+        p[a] = o.p;
+        if (store) {
+          d.delta[a * cap + i] = o.dl;
+          d.pos[a * cap + i] = o.p;
+          d.vbuf[a * cap + i] = o.vb;
+        }
       } else {
-        p[a] = d.pos[a * cap + i];
+        p[a] = x;
+        d.vbuf[a * cap + i] = drift_axis(R, a, x, v, ac, 0.0).vb;
       }
-      d.vbuf[a * cap + i] = v + 0.5 * dt * ac;
       fin = fin && isfinite(p[a]) && isfinite(v);
     }
     if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
@@ -119,7 +143,11 @@ __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict
 void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st, bool tick) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, keys ? 1 : 0,
-                     predict_only ? 1 : 0, tick ? 1 : 0);
+                     predict_only ? 1 : 0, tick ? 1 : 0, 1);
+}
+void launch_step_keys(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0);
 }
 
 __global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
@@ -301,7 +329,11 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
 // other master buffer set, and writes the FP32 copies the pair passes read:
 // position inside the bead's cell (FP64 difference, then rounded) with
 // id << 2 | type in .w, and the velocity the force uses.
-__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin) {
+// drift: the master set holds the state before this step's drift; the bead
+// is drifted here (same arithmetic as k_drift) and written once, drifted,
+// to its new slot.
+__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin,
+                                                 int drift) {
   if (skin && *d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
@@ -318,12 +350,19 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   double xs[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double x = d.pos[a * cap + i];
+    double x = d.pos[a * cap + i];
+    double vb, dl = d.delta[a * cap + i];
+    if (drift) {
+      const DriftAxis o = drift_axis(R, a, x, d.vel[a * cap + i], (double)d.acc[a * cap + i], dl);
+      x = o.p;
+      dl = o.dl;
+      vb = o.vb;
+    } else {
+      vb = d.vbuf[a * cap + i];
+    }
     xs[a] = x;
-    const double vb = d.vbuf[a * cap + i];
     d.alt.pos[a * cap + s] = x;
     d.alt.vbuf[a * cap + s] = vb;
-    const double dl = d.delta[a * cap + i];
     d.alt.delta[a * cap + s] = dl;
     // skin mode: the displacement test of the coming steps starts here
     if (skin) d.dref[a * cap + s] = dl;
@@ -351,7 +390,11 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
-                     full ? 1 : 0, 0);
+                     full ? 1 : 0, 0, 0);
+}
+void launch_step_permute(const DevState& d, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1);
 }
 
 // chain bead slots follow the beads to their new slots
@@ -666,7 +709,6 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
   if (i >= d.n) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
-  const double dt = R.dt;
   const uint32_t cap = d.cap;
   const uint32_t c = d.scell[i] - R.cell_off;
   const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
@@ -677,14 +719,12 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double vv = d.vel[a * cap + i];
-    const double ac = (double)d.acc[a * cap + i];
-    const double dr = dt * (vv + 0.5 * dt * ac);
-    const double tr = d.pos[a * cap + i] + dr;
-    const double dl = d.delta[a * cap + i] + dr;
+    const DriftAxis o = drift_axis(R, a, d.pos[a * cap + i], vv, (double)d.acc[a * cap + i],
+                                   d.delta[a * cap + i]);
+    const double dl = o.dl, vb = o.vb;
+    p[a] = o.p;
     d.delta[a * cap + i] = dl;
-    p[a] = tr - floor(tr * R.inv_box[a]) * R.box[a];
     d.pos[a * cap + i] = p[a];
-    const double vb = vv + 0.5 * dt * ac;
     d.vbuf[a * cap + i] = vb;
     fin = fin && isfinite(p[a]) && isfinite(vv);
     const double e = dl - d.dref[a * cap + i];
@@ -724,7 +764,7 @@ void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
 }
 void launch_permute_gated(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0);
 }
 
 // The rebuild's permutation wrote the other buffer set; copy it back so the

@@ -38,7 +38,7 @@ struct DevState : Master {
   uint32_t ncell;
   // FP32 copies in cell order (same slots as the master storage)
   float4* spos;         // local coordinates in the bead's cell, .w = id<<2 | type
-  float4* svel;         // predicted velocity
+  float4* svel;         // predicted velocity; .w = density when n_dens == 1
   uint32_t* scell;      // global cell of each slot at the last rebuild
   float* sdens;         // [nd][cap] density
   uint32_t* nlist;      // wave-tiled pair list [cap/64][W/4][64][4]
@@ -79,6 +79,11 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st,
                   bool tick = false);
 void launch_keys(const DevState& d, hipStream_t st);
+// the step's first kernel: clock, and cell key + histogram of each bead's
+// drifted position (the state itself is drifted by launch_step_permute)
+void launch_step_keys(const DevState& d, hipStream_t st);
+// the step's permutation: drift + move into the other set in cell order
+void launch_step_permute(const DevState& d, hipStream_t st);
 void launch_scan_cells(const DevState& d, hipStream_t st);
 void launch_scatter(const DevState& d, hipStream_t st);
 void launch_cell_sort(const DevState& d, hipStream_t st);
