@@ -242,7 +242,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.bsum, (size_t)div_up(cells, 1024) + 1);
   ALLOC(d.perm, cap64);
   ALLOC(d.spos, cap64);
-  ALLOC(d.svel, cap64);
+  ALLOC(d.sb, cap64);
   ALLOC(d.scell, cap64);
   ALLOC(d.sdens, (size_t)c->nd * cap64);
   ALLOC(d.nlist, (size_t)cap64 * c->tune.nlist_width);

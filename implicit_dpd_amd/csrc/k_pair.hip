@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
     for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
     // one density: also in the velocity copy's spare lane, so the force
     // pass gets a neighbour's velocity and density in one 16-byte load
-    if (ND == 1) reinterpret_cast<float*>(d.svel + s)[3] = rho[0];
+    if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
     d.ncount[s] = cnt;
   }
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
   }
 #pragma unroll
   for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
-  if (ND == 1) reinterpret_cast<float*>(d.svel + s)[3] = rho[0];
+  if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
   d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
 }
@@ -361,8 +361,8 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const uint32_t seed3 = R.pair_seed + *d.d_time;
-  const float4 pi = d.spos[s];
-  const float4 vi = d.svel[s];
+  const float4 pi = d.sb[s].pos;
+  const float4 vi = d.sb[s].vel;
   const uint32_t wi = __float_as_uint(pi.w);
   const uint32_t idi = wi >> 2, ti = NS == 1 ? 0u : (wi & 3u), ti3 = 3u * ti;
   float rho_i[ND];
@@ -390,8 +390,8 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   auto pair = [&](uint32_t e) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
-    const float4 pj = d.spos[t];
-    const float4 vj = d.svel[t];
+    const float4 pj = d.sb[t].pos;
+    const float4 vj = d.sb[t].vel;
     const float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
     const float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
     const float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;

@@ -383,8 +383,10 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   const uint32_t ch = d.chain[i];
   d.alt.chain[s] = ch;
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(xs[0], xs[1], xs[2], 0.0);
-  d.spos[s] = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
-  d.svel[s] = make_float4(v[0], v[1], v[2], 0.f);
+  const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
+  d.spos[s] = ps;
+  d.sb[s].pos = ps;
+  d.sb[s].vel = make_float4(v[0], v[1], v[2], 0.f);
   d.scell[s] = key;
 }
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
@@ -681,8 +683,9 @@ __global__ void __launch_bounds__(256) k_head_conv(DevState d) {
     d.prop[i] = (uint32_t)R.conv_to;
     // the type also rides in the FP32 copy (read by the pair passes before
     // the next rebuild in skin mode)
-    reinterpret_cast<float*>(d.spos + i)[3] =
-        __uint_as_float((d.id[i] << 2) | ((uint32_t)R.conv_to & 3u));
+    const float w = __uint_as_float((d.id[i] << 2) | ((uint32_t)R.conv_to & 3u));
+    reinterpret_cast<float*>(&d.spos[i])[3] = w;
+    reinterpret_cast<float*>(&d.sb[i].pos)[3] = w;
   }
 }
 void launch_head_conversion(const DevState& d, hipStream_t st) {
@@ -737,11 +740,13 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
   }
   if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
   if (!(disp <= d.disp2)) *d.rebuild = 1u;  // every writer stores the same value
-  float* sp = reinterpret_cast<float*>(d.spos + i);  // .w (id, type) stays
-  sp[0] = l[0];
-  sp[1] = l[1];
-  sp[2] = l[2];
-  d.svel[i] = make_float4(v[0], v[1], v[2], 0.f);
+  // .w (id, type) stays
+  float* sp = reinterpret_cast<float*>(&d.spos[i]);
+  float* sq = reinterpret_cast<float*>(&d.sb[i].pos);
+  sp[0] = sq[0] = l[0];
+  sp[1] = sq[1] = l[1];
+  sp[2] = sq[2] = l[2];
+  d.sb[i].vel = make_float4(v[0], v[1], v[2], 0.f);
   const uint32_t ch = d.chain[i];
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(p[0], p[1], p[2], 0.0);
 }

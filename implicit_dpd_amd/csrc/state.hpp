@@ -23,6 +23,15 @@ struct Master {
   uint32_t* chain;  // chain index << 2 | position in the chain, or ~0u
 };
 
+// The force pass's per-bead record, 32 bytes: a neighbour gather reads one
+// 32-byte piece of a cache line instead of pieces of two lines.  (The
+// density pass streams the positions alone, 16 bytes apart, so they are
+// also kept in their own array.)
+struct SBead {
+  float4 pos;  // local coordinates in the bead's cell, .w = id << 2 | type
+  float4 vel;  // predicted velocity; .w = density when n_dens == 1
+};
+
 struct DevState : Master {
   Master alt;       // the other buffer set, target of the next permutation
   uint32_t cap;
@@ -37,8 +46,8 @@ struct DevState : Master {
   uint32_t* inv_perm;   // [cap] old slot -> new slot
   uint32_t ncell;
   // FP32 copies in cell order (same slots as the master storage)
-  float4* spos;         // local coordinates in the bead's cell, .w = id<<2 | type
-  float4* svel;         // predicted velocity; .w = density when n_dens == 1
+  float4* spos;         // [cap] the positions alone, for the cell and list walks
+  SBead* sb;            // [cap] position + velocity, for the force pass
   uint32_t* scell;      // global cell of each slot at the last rebuild
   float* sdens;         // [nd][cap] density
   uint32_t* nlist;      // wave-tiled pair list [cap/64][W/4][64][4]
