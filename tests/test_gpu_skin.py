@@ -181,18 +181,27 @@ def test_skin_chem_event_after_run(engine_mod, oracle):
     e.close()
 
 
-def test_skin_needs_three_cells_per_edge(engine_mod):
+def test_skin_small_box(engine_mod, oracle):
+    """A box edge of three list radii (one x super-cell spanning the whole
+    box, so beads crossing the periodic boundary between builds must keep
+    coordinates continuous with their build cell), and the size limit."""
     box = (5.0, 6.0, 6.0)
     p = configs.groot_warren_params(box, pair_seed=1)
     s = configs.random_fluid(540, box, 1)
     e = _engine(engine_mod, p, s, 0.0)
-    e.set_tuning(verlet_skin=0.6)  # floor(5 / 1.6) = 3 cells: accepted
+    e.set_tuning(verlet_skin=0.6)  # 5 >= 3 (1 + 0.6): accepted
     with pytest.raises(MdpdError) as ex:
-        e.set_tuning(verlet_skin=0.7)  # floor(5 / 1.7) = 2
+        e.set_tuning(verlet_skin=0.7)  # 5 < 3 (1 + 0.7)
     assert ex.value.code == MDPD_ERR_ARG
     with pytest.raises(MdpdError):
         e.set_tuning(verlet_skin=0.01)
     # the failed calls left the accepted tuning in place
     assert abs(e.tuning().verlet_skin - 0.6) < 1e-7
-    e.run(4)
+    reused = 0
+    for k in range(16):
+        b = e.skin_builds()
+        e.run(1)
+        reused += e.skin_builds() == b
+        _check_pairs(e, oracle, p)
+    assert reused > 0
     e.close()
