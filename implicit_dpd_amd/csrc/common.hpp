@@ -51,6 +51,10 @@ struct Replica {
   float mb[3][3][3];
 };
 
+// x cells are at least 1 / kXWin list radii wide: a bead's x window is the
+// 2 kXWin + 1 cells around its own.
+constexpr int kXWin = 2;
+
 // Guard band on r^2 (FP32 local-coordinate differences carry |error| of a
 // few 1e-7); inside it the pair decision is redone in FP64 from the master
 // positions with the oracle's minimum-image expression.

@@ -87,12 +87,12 @@ int check_params(const mdpd_params& p, std::string& why) {
 
 // Cell grid of a replica for a list radius c (the cutoff, or 1 + skin in
 // skin mode): y and z cells at least c wide (a three-cell window); x cells
-// at least c / 2 wide (a five-cell window, 2.5 c long instead of 3 c),
-// grouped into super-cells of gx cells whose width ws lies in
-// [max(4, 2.5 c), 2 max(4, 2.5 c)) or is the whole box.  FP32 x coordinates
-// are relative to the super-cell origin: below 2 ws in magnitude, so their
-// rounding stays far inside the guard band, and a five-cell window spans at
-// most two super-cells.
+// at least c / kXWin wide (a window of 2 kXWin + 1 cells, (2 + 1 / kXWin) c
+// long instead of 3 c), grouped into super-cells of gx cells whose width ws
+// lies in [max(4, 2.5 c), 2 max(4, 2.5 c)) or is the whole box.  FP32 x
+// coordinates are relative to the super-cell origin: below 2 ws in
+// magnitude, so their rounding stays far inside the guard band, and gx >=
+// 2 kXWin + 1, so a window spans at most two super-cells.
 void set_cells(Replica& r, double c) {
   for (int a = 1; a < 3; ++a) {
     r.nc[a] = (int)std::floor(r.box[a] / c);
@@ -101,7 +101,7 @@ void set_cells(Replica& r, double c) {
   const double wmin = std::max(4.0, 2.5 * c);
   const int ns = std::max(1, (int)std::floor(r.box[0] / wmin));
   const double ws = r.box[0] / ns;
-  r.gx = (int)std::floor(ws / (0.5 * c));
+  r.gx = (int)std::floor(ws / (c / kXWin));
   r.nc[0] = ns * r.gx;
   r.h[0] = r.box[0] / r.nc[0];
   r.ws = r.gx * r.h[0];

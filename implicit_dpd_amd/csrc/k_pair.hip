@@ -84,15 +84,15 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
-  // x: cells at least half a cutoff wide, so the window is the five cells
-  // [cx - 2, cx + 2]; x coordinates are relative to the bead's super-cell
+  // x: cells at least 1 / kXWin cutoff wide, so the window is the cells
+  // [cx - kXWin, cx + kXWin]; x coordinates are relative to the bead's super-cell
   // (gx cells), so the window's offset term changes only where it crosses
   // a super-cell boundary (at most one): XA before the boundary slot, XB
   // from it on.
   const int gx = R.gx;
   const float wsf = R.wsf;
   const int sxi = cx / gx;
-  const int u0 = cx - 2, u1 = cx + 2;
+  const int u0 = cx - kXWin, u1 = cx + kXWin;
   const int sA = u0 < 0 ? -1 : u0 / gx, sB = u1 / gx;
   const int bnd = sB * gx;  // first cell of the window's right super-cell
   const float XA = pi.x - (float)(sA - sxi) * wsf, XB = pi.x - (float)(sB - sxi) * wsf;
@@ -560,10 +560,11 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
   const int gx = R.gx, sxi = cx / gx;
   for (int oz = -1; oz <= 1; ++oz)
     for (int oy = -1; oy <= 1; ++oy)
-      for (int ox = -2; ox <= 2; ++ox) {
-        // x: five cells of at least half a cutoff, offsets in super-cells
+      for (int ox = -kXWin; ox <= kXWin; ++ox) {
+        // x: the window of cells at least 1 / kXWin cutoff wide, offsets in
+        // super-cells
         const int u = cx + ox;
-        const int su = u < 0 ? -1 : u / gx;
+        const int su = u < 0 ? -1 : u / gx;  // (|u| < gx for u < 0)
         const int qx = (u + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
         const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
         const float bx = pi.x - (float)(su - sxi) * R.wsf, by = pi.y - (float)oy * R.hf[1],
