@@ -695,10 +695,6 @@ void launch_head_conversion(const DevState& d, hipStream_t st) {
   hipLaunchKernelGGL(k_head_conv, grid1(d.n, 256), dim3(256), 0, st, d);
 }
 
-__global__ void k_time_inc(uint32_t* t) { *t += 1u; }
-void launch_time_inc(uint32_t* d_time, hipStream_t st) {
-  hipLaunchKernelGGL(k_time_inc, dim3(1), dim3(1), 0, st, d_time);
-}
 
 // ---------------------------------------------------------------- skin mode
 // drift_and_predict (driftkick.hpp:5-30, the same FP64 expressions as

@@ -119,7 +119,6 @@ void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_o
 void launch_chainpos_fill(const DevState& d, hipStream_t st);
 void launch_cm_drift(const DevState& d, hipStream_t st);
 void launch_head_conversion(const DevState& d, hipStream_t st);
-void launch_time_inc(uint32_t* d_time, hipStream_t st);
 // skin mode (launches of the rebuild kernels return at once unless
 // *d.rebuild is set)
 void launch_drift_skin(const DevState& d, hipStream_t st);
