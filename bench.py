@@ -119,15 +119,6 @@ def cpu_baseline(p, s, topo, budget_s, max_steps):
                       f"system, {steps} steps after 1 warm-up step, {dt:.1f} s"}
 
 
-def cells_of(box):
-    """Cells of the engine's grid for a box: y and z cells at least one
-    cutoff wide, x cells at least half a cutoff in super-cells of width
-    >= 4."""
-    ns = max(1, int(math.floor(box[0] / 4.0)))
-    nx = ns * int(math.floor((box[0] / ns) / 0.5))
-    return nx * int(math.floor(box[1])) * int(math.floor(box[2]))
-
-
 class Workload:
     """One benchmark system: parameters, initial state and its description."""
 
@@ -140,7 +131,8 @@ class Workload:
         self.sizes = [len(x["pos"]) for x in slist]
         self.n = sum(self.sizes)
         self.volume = sum(q.box[0] * q.box[1] * q.box[2] for q in plist)
-        self.ncell = sum(cells_of(q.box) for q in plist)
+        self.ncell = sum(max(int(math.floor(q.box[0])), 1) * max(int(math.floor(q.box[1])), 1) *
+                         max(int(math.floor(q.box[2])), 1) for q in plist)
         self.nd = plist[0].n_dens
 
     def start(self, Engine, e=None, warmup=1, skin=0.0):

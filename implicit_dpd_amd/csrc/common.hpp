@@ -23,16 +23,13 @@ enum : uint32_t {
 struct Replica {
   double box[3];
   double inv_box[3];
-  double h[3];          // cell edge = box / nc (x: at least half a cutoff)
+  double h[3];          // cell edge = box / nc
   double inv_h[3];
-  double ws;            // x super-cell width = gx * h[0]
   double dt;
   double rd2;           // density cutoff^2 (FP64 decisions)
   double conv_p;
   double cf_s, ibond;   // bond spring, 1/bond_leng
   int32_t nc[3];
-  int32_t gx;           // x cells per super-cell (FP32 x coordinates are
-                        // relative to the super-cell origin)
   uint32_t cell_off;    // first global cell of this replica
   uint32_t bead_off;    // first bead slot
   uint32_t n_beads;
@@ -42,7 +39,6 @@ struct Replica {
   int32_t conv_from, conv_to;
   int32_t n_species, n_dens, envelope;
   float hf[3];
-  float wsf;            // ws in FP32
   float rd, inv_rd, crho, inv_rdm, arc;
   float rd2f;
   int32_t rdm_is_one;
@@ -50,10 +46,6 @@ struct Replica {
   float a0[3][3], c1[3][3], sigma[3][3], gamma[3][3];
   float mb[3][3][3];
 };
-
-// x cells are at least 1 / kXWin list radii wide: a bead's x window is the
-// 2 kXWin + 1 cells around its own.
-constexpr int kXWin = 2;
 
 // Guard band on r^2 (FP32 local-coordinate differences carry |error| of a
 // few 1e-7); inside it the pair decision is redone in FP64 from the master

@@ -85,28 +85,12 @@ int check_params(const mdpd_params& p, std::string& why) {
   return MDPD_OK;
 }
 
-// Cell grid of a replica for a list radius c (the cutoff, or 1 + skin in
-// skin mode): y and z cells at least c wide (a three-cell window); x cells
-// at least c / kXWin wide (a window of 2 kXWin + 1 cells, (2 + 1 / kXWin) c
-// long instead of 3 c), grouped into super-cells of gx cells whose width ws
-// lies in [max(4, 2.5 c), 2 max(4, 2.5 c)) or is the whole box.  FP32 x
-// coordinates are relative to the super-cell origin: below 2 ws in
-// magnitude, so their rounding stays far inside the guard band, and gx >=
-// 2 kXWin + 1, so a window spans at most two super-cells.
-void set_cells(Replica& r, double c) {
-  for (int a = 1; a < 3; ++a) {
-    r.nc[a] = (int)std::floor(r.box[a] / c);
-    r.h[a] = r.box[a] / r.nc[a];
-  }
-  const double wmin = std::max(4.0, 2.5 * c);
-  const int ns = std::max(1, (int)std::floor(r.box[0] / wmin));
-  const double ws = r.box[0] / ns;
-  r.gx = (int)std::floor(ws / (c / kXWin));
-  r.nc[0] = ns * r.gx;
-  r.h[0] = r.box[0] / r.nc[0];
-  r.ws = r.gx * r.h[0];
-  r.wsf = (float)r.ws;
+// cell grid of a replica: the most cells per edge that are at least
+// min_cell wide (the cutoff, or 1 + skin in skin mode)
+void set_cells(Replica& r, double min_cell) {
   for (int a = 0; a < 3; ++a) {
+    r.nc[a] = (int)std::floor(r.box[a] / min_cell);
+    r.h[a] = r.box[a] / r.nc[a];
     r.inv_h[a] = 1.0 / r.h[a];
     r.hf[a] = (float)r.h[a];
   }
@@ -799,7 +783,7 @@ int mdpd_set_tuning(mdpd_ctx* c, const mdpd_tuning* t) {
       for (int a = 0; a < 3; ++a)
         if (std::floor(c->R[r].box[a] / (1.0 + sk)) < 3.0)
           return fail(c, MDPD_ERR_ARG, "replica " + std::to_string(r) +
-                                           ": a box edge is shorter than 3 (1 + verlet_skin)");
+                                           ": a box edge holds fewer than 3 cells of 1 + verlet_skin");
   if (t->nlist_width != c->tune.nlist_width) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     const uint32_t cap64 = div_up(c->cap, 64) * 64;

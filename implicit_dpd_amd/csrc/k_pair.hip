@@ -42,6 +42,22 @@ __device__ __forceinline__ double fp64_r2(const DevState& d, const Replica& R, u
   return x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
 }
 
+// displacement pos_i - pos_j, minimum image, FP64
+__device__ __forceinline__ void fp64_dvec(const DevState& d, const Replica& R, uint32_t i,
+                                          uint32_t j, double v[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double x = d.pos[a * d.cap + i] - d.pos[a * d.cap + j];
+    if (x > 0.5 * R.box[a]) x -= R.box[a];
+    else if (x < -0.5 * R.box[a]) x += R.box[a];
+    v[a] = x;
+  }
+}
+
+// force pass: pairs closer than sqrt(kClose2) take their displacement in FP64
+constexpr float kClose2 = 1.0e-3f;
+
 // Pair decision shared by every pass: returns (r^2 < cut2) as the FP64
 // oracle decides it.  r2 is the FP32 estimate.
 __device__ __forceinline__ bool decide(float r2, float cut2f, double cut2, const DevState& d,
@@ -70,7 +86,7 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   // replica constants in registers for the loops below
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
   const uint32_t cell_off = R.cell_off;
-  const float hy = R.hf[1], hz = R.hf[2];
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
   const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
   const float cut_hi = BUILD ? d.vcut2 : fmaxf(1.0f, rd2f) + kGuard;
   const float4 pi = d.spos[s];
@@ -84,26 +100,13 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
-  // x: cells at least 1 / kXWin cutoff wide, so the window is the cells
-  // [cx - kXWin, cx + kXWin]; x coordinates are relative to the bead's super-cell
-  // (gx cells), so the window's offset term changes only where it crosses
-  // a super-cell boundary (at most one): XA before the boundary slot, XB
-  // from it on.
-  const int gx = R.gx;
-  const float wsf = R.wsf;
-  const int sxi = cx / gx;
-  const int u0 = cx - kXWin, u1 = cx + kXWin;
-  const int sA = u0 < 0 ? -1 : u0 / gx, sB = u1 / gx;
-  const int bnd = sB * gx;  // first cell of the window's right super-cell
-  const float XA = pi.x - (float)(sA - sxi) * wsf, XB = pi.x - (float)(sB - sxi) * wsf;
-  const uint32_t CXA = (uint32_t)(1 + sA - sxi) << kCodeShift;
-  const uint32_t CXB = (uint32_t)(1 + sB - sxi) << kCodeShift;
-  // the window crosses the periodic boundary: two runs (the boundary is
-  // then the super-cell boundary bnd = 0 or nx)
-  const bool wrap = u0 < 0 || u1 >= nx;
+  // The bead's x offset term for the left, own and right neighbour cell;
+  // a candidate picks one by comparing its slot with two run boundaries.
+  const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
   // Pair-list entries are buffered in a three-deep shift register and
   // written as whole 16-byte chunks of four.
   uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
+  const bool wrap = !(cx >= 1 && cx <= nx - 2);
   const uint32_t kInf = 0xFFFFFFFFu;
   for (int oz = -1; oz <= 1; ++oz) {
     int qz = cz + oz;
@@ -115,39 +118,34 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       qy += (qy < 0) ? ny : 0;
       qy -= (qy >= ny) ? ny : 0;
       const float by = pi.y - (float)oy * hy;
-      // The window's cells are consecutive in cell order, hence one
-      // contiguous run of slots (two runs where the row wraps).
+      // The three x-neighbour cells of a row are consecutive in cell order,
+      // hence one contiguous run of slots (two runs where the row wraps).
       const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t* S = d.cell_start + rowq;
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      const uint32_t CA = cyz | CXA, CB = cyz | CXB;
+      const uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
       for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
-        // run [ta, tb); slots >= tb2 take the right super-cell's offset
-        uint32_t ta, tb, tb2;
-        if (!wrap) {
-          ta = S[u0];
-          tb = S[u1 + 1];
-          tb2 = sA != sB ? S[bnd] : kInf;
-        } else if (pc == 0) {  // cells [u0, bnd - 1], wrapped
-          ta = S[u0 < 0 ? u0 + nx : u0];
-          tb = S[bnd - 1 < 0 ? nx : bnd];
-          tb2 = kInf;
-        } else {  // cells [bnd, u1], wrapped
-          ta = S[bnd >= nx ? 0 : bnd];
-          tb = S[(u1 >= nx ? u1 - nx : u1) + 1];
-          tb2 = 0;
+        // run [ta, tb); slots >= b2 are in the right cell, >= b1 the own cell
+        uint32_t ta, tb, b1, b2;
+        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
+        else if (cx == 0) {
+          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; b1 = kInf; b2 = kInf; }
+          else { ta = S[0]; tb = S[2]; b1 = 0; b2 = S[1]; }
+        } else {
+          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
+          else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
         }
         for (uint32_t t = ta; t < tb; ++t) {
           // (other waves hide the load latency; a software prefetch costs
           // six register moves per candidate)
           const float4 pj = d.spos[t];
-          const bool gb = t >= tb2;
-          const float bx = gb ? XB : XA;
+          const bool g2 = t >= b2, g1 = t >= b1;
+          const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (BUILD) {
             if (r2 >= cut_hi || t == s) continue;
-            const uint32_t e = t | (gb ? CB : CA);
+            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
             if ((cnt & 3u) == 3u) {
               if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
             } else {
@@ -186,7 +184,7 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           if (in_rc) {
             // the register window always shifts; every fourth entry the
             // window plus the entry that fell out go out as one chunk
-            const uint32_t e = t | (gb ? CB : CA);
+            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
             const uint32_t out = e0;
             e0 = e1;
             e1 = e2;
@@ -251,8 +249,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
-  // x offsets are whole super-cells, y and z offsets whole cells
-  const float hx = R.wsf, hy = R.hf[1], hz = R.hf[2];
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
   const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
   const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
   const float4 pi = d.spos[s];
@@ -397,8 +394,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
 #pragma unroll
     for (int k = 0; k < ND; ++k) MB[k][q] = R.mb[ti][q][k];
   }
-  // x offsets are whole super-cells, y and z offsets whole cells
-  const float hx = R.wsf, hy = R.hf[1], hz = R.hf[2];
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
   const float arc = R.arc, inv_rdm = R.inv_rdm;
   const bool rdm1 = R.rdm_is_one != 0;
   const bool has_c1 = R.has_c1 != 0;
@@ -412,10 +408,23 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     const uint32_t code = e >> kCodeShift;
     const float4 pj = d.sb[t].pos;
     const float4 vj = d.sb[t].vel;
-    const float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
-    const float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
-    const float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+    float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
+    float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < kClose2) {
+      // a near-coincident pair: the FP32 coordinates' absolute error (a few
+      // 1e-7) is too coarse for the direction of so short a displacement;
+      // take it from the FP64 master positions (rare)
+      uint32_t tt = t;
+      asm volatile("" : "+v"(tt));
+      double v[3];
+      fp64_dvec(d, R, s, tt, v);
+      dx = (float)v[0];
+      dy = (float)v[1];
+      dz = (float)v[2];
+      r2 = dx * dx + dy * dy + dz * dz;
+    }
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float rr = r2 * rinv;
     const uint32_t wj = __float_as_uint(pj.w);
@@ -557,17 +566,12 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
   const int cx = (int)(c % (uint32_t)nx), cy = (int)((c / (uint32_t)nx) % (uint32_t)ny),
             cz = (int)(c / (uint32_t)(nx * ny));
-  const int gx = R.gx, sxi = cx / gx;
   for (int oz = -1; oz <= 1; ++oz)
     for (int oy = -1; oy <= 1; ++oy)
-      for (int ox = -kXWin; ox <= kXWin; ++ox) {
-        // x: the window of cells at least 1 / kXWin cutoff wide, offsets in
-        // super-cells
-        const int u = cx + ox;
-        const int su = u < 0 ? -1 : u / gx;  // (|u| < gx for u < 0)
-        const int qx = (u + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
+      for (int ox = -1; ox <= 1; ++ox) {
+        const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
         const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-        const float bx = pi.x - (float)(su - sxi) * R.wsf, by = pi.y - (float)oy * R.hf[1],
+        const float bx = pi.x - (float)ox * R.hf[0], by = pi.y - (float)oy * R.hf[1],
                     bz = pi.z - (float)oz * R.hf[2];
         for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) {
           if (t == s) continue;

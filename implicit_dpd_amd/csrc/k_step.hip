@@ -344,9 +344,7 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   const uint32_t key = d.key[i];
   const uint32_t c = key - R.cell_off;
   const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
-  // x relative to the origin of the bead's super-cell, y and z to its cell
-  const uint32_t cc[3] = {(c % nx) / (uint32_t)R.gx * (uint32_t)R.gx, (c / nx) % ny,
-                          c / (nx * ny)};
+  const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
   const uint32_t cap = d.cap;
   float l[3], v[3];
   double xs[3];
@@ -713,9 +711,7 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
   const uint32_t cap = d.cap;
   const uint32_t c = d.scell[i] - R.cell_off;
   const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
-  // x relative to the origin of the bead's super-cell, y and z to its cell
-  const uint32_t cell[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
-  const uint32_t cc[3] = {cell[0] / (uint32_t)R.gx * (uint32_t)R.gx, cell[1], cell[2]};
+  const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
   double p[3], disp = 0.0;
   float l[3], v[3];
   bool fin = true;
@@ -732,14 +728,9 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
     fin = fin && isfinite(p[a]) && isfinite(vv);
     const double e = dl - d.dref[a * cap + i];
     disp += e * e;
-    // the image nearest the centre of the bead's build cell: the bead was
-    // inside that cell at the build and has moved less than skin / 2 since
-    // (a super-cell may span the whole box, so a half-box threshold on the
-    // coordinate itself would be ambiguous)
-    const double org = (double)cc[a] * R.h[a];
-    const double ref = ((double)cell[a] + 0.5) * R.h[a] - org;
-    double x = p[a] - org;
-    x -= R.box[a] * rint((x - ref) * R.inv_box[a]);
+    double x = p[a] - (double)cc[a] * R.h[a];
+    if (x > 0.5 * R.box[a]) x -= R.box[a];
+    else if (x < -0.5 * R.box[a]) x += R.box[a];
     l[a] = (float)x;
     v[a] = (float)vb;
   }

@@ -182,9 +182,9 @@ def test_skin_chem_event_after_run(engine_mod, oracle):
 
 
 def test_skin_small_box(engine_mod, oracle):
-    """A box edge of three list radii (one x super-cell spanning the whole
-    box, so beads crossing the periodic boundary between builds must keep
-    coordinates continuous with their build cell), and the size limit."""
+    """A box edge of three list radii (beads crossing the periodic boundary
+    between builds keep coordinates continuous with their build cell), and
+    the size limit."""
     box = (5.0, 6.0, 6.0)
     p = configs.groot_warren_params(box, pair_seed=1)
     s = configs.random_fluid(540, box, 1)
