@@ -26,6 +26,7 @@ MDPD_ERR_UNSUPPORTED = 10
 HYPHIL, HYPHOB, SOLVENT = 0, 1, 2
 FPDPD_STRIDE = 232
 UNIT_NONE = 0xFFFFFFFF
+MDPD_TUNE_SKIN = 1
 
 D33 = (C.c_double * 3) * 3
 D333 = ((C.c_double * 3) * 3) * 3
@@ -174,6 +175,7 @@ PROTOTYPES = {
     "mdpd_set_profiling": (C.c_int, [VP, C.c_int32]),
     "mdpd_phase_times": (C.c_int, [VP, _DP, C.c_int32, P(C.c_int32), C.c_char_p, C.c_size_t]),
     "mdpd_autotune": (C.c_int, [VP, C.c_uint32, P(MdpdTuning)]),
+    "mdpd_autotune_ex": (C.c_int, [VP, C.c_uint32, C.c_uint32, P(MdpdTuning)]),
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
     "mdpd_run_timed": (C.c_int, [VP, C.c_uint32, _DP]),

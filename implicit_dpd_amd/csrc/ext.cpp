@@ -153,15 +153,20 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
 // Times every (block_scan, block_force) pair on the current state for
 // `nsteps` steps and keeps the fastest.  The state is restored after every
 // candidate; per-bead sums run in a fixed order, so every candidate gives
-// the same bits.  With MDPD_TUNE_CACHE set, results are kept in that file.
-int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
+// the same bits.  With MDPD_TUNE_SKIN in flags it then also times the
+// neighbour-list skins 0.1, 0.2 and 0.3 (cells of 1 + skin; those the boxes
+// allow) against the every-step cell list, with the best block sizes.
+// With MDPD_TUNE_CACHE set, results are kept in that file.
+int mdpd_autotune_ex(mdpd_ctx* c, uint32_t nsteps, uint32_t flags, mdpd_tuning* out) {
   if (!c) return MDPD_ERR_ARG;
   if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "autotune between drift and kick");
+  if (flags & ~(uint32_t)MDPD_TUNE_SKIN) return fail(c, MDPD_ERR_ARG, "unknown autotune flags");
   DevState& d = c->d;
   char keybuf[256];
-  snprintf(keybuf, sizeof(keybuf), "n=%u,ncell=%u,nrep=%d,ns=%d,nd=%d,env=%d,namp=%u,W=%d,skin=%g",
-           d.n, d.ncell, c->n_rep, c->ns, c->nd, c->env, d.n_amp, d.W, (double)c->tune.verlet_skin);
+  snprintf(keybuf, sizeof(keybuf),
+           "n=%u,ncell=%u,nrep=%d,ns=%d,nd=%d,env=%d,namp=%u,W=%d,skin=%g,flags=%u", d.n, d.ncell,
+           c->n_rep, c->ns, c->nd, c->env, d.n_amp, d.W, (double)c->tune.verlet_skin, flags);
   const std::string key(keybuf);
   const char* cache = std::getenv("MDPD_TUNE_CACHE");
   if (cache) {
@@ -171,10 +176,12 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
       std::istringstream ss(line);
       std::string k;
       int bs = 0, bf = 0;
-      if ((ss >> k >> bs >> bf) && k == key) {
+      float sk = 0.0f;
+      if ((ss >> k >> bs >> bf >> sk) && k == key) {
         mdpd_tuning t = c->tune;
         t.block_scan = bs;
         t.block_force = bf;
+        t.verlet_skin = sk;
         int rc = mdpd_set_tuning(c, &t);
         if (rc) return rc;
         if (out) *out = c->tune;
@@ -192,76 +199,103 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
   HIPCHK(c, hipMalloc(&snapf, 2 * n3 * sizeof(float)));
   HIPCHK(c, hipMalloc(&snapu, (6 * (size_t)d.cap + 4 * (size_t)c->amp_cap) * sizeof(uint32_t)));
   const uint32_t t0 = c->time;
+  // (d is re-read through c->d inside: the storage parity may flip)
   auto save = [&]() -> int {
-    HIPCHK(c, hipMemcpyAsync(snap, d.pos, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(snap + n3, d.vel, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(snap + 2 * n3, d.vbuf, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(snap + 3 * n3, d.delta, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(snapf, d.acc, n3 * 4, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(snapf + n3, d.press, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    DevState& e = c->d;
+    HIPCHK(c, hipMemcpyAsync(snap, e.pos, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + n3, e.vel, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + 2 * n3, e.vbuf, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snap + 3 * n3, e.delta, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snapf, e.acc, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(snapf + n3, e.press, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     {
       // every per-slot array the storage permutation moves, chain included
-      uint32_t* const src[6] = {d.prop, d.id, d.amp_id, d.unit, d.orig, d.chain};
+      uint32_t* const src[6] = {e.prop, e.id, e.amp_id, e.unit, e.orig, e.chain};
       for (int q = 0; q < 6; ++q)
-        HIPCHK(c, hipMemcpyAsync(snapu + (size_t)q * d.cap, src[q], (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
-      HIPCHK(c, hipMemcpyAsync(snapu + 6 * (size_t)d.cap, d.topo, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(snapu + (size_t)q * e.cap, src[q], (size_t)e.cap * 4, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(snapu + 6 * (size_t)e.cap, e.topo, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
     }
     return MDPD_OK;
   };
   auto restore = [&]() -> int {
-    HIPCHK(c, hipMemcpyAsync(d.pos, snap, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(d.vel, snap + n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(d.vbuf, snap + 2 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(d.delta, snap + 3 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(d.acc, snapf, n3 * 4, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(d.press, snapf + n3, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    DevState& e = c->d;
+    HIPCHK(c, hipMemcpyAsync(e.pos, snap, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.vel, snap + n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.vbuf, snap + 2 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.delta, snap + 3 * n3, n3 * 8, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.acc, snapf, n3 * 4, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.press, snapf + n3, n3 * 4, hipMemcpyDeviceToDevice, c->st));
     {
-      uint32_t* const dst[6] = {d.prop, d.id, d.amp_id, d.unit, d.orig, d.chain};
+      uint32_t* const dst[6] = {e.prop, e.id, e.amp_id, e.unit, e.orig, e.chain};
       for (int q = 0; q < 6; ++q)
-        HIPCHK(c, hipMemcpyAsync(dst[q], snapu + (size_t)q * d.cap, (size_t)d.cap * 4, hipMemcpyDeviceToDevice, c->st));
-      HIPCHK(c, hipMemcpyAsync(d.topo, snapu + 6 * (size_t)d.cap, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(dst[q], snapu + (size_t)q * e.cap, (size_t)e.cap * 4, hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(e.topo, snapu + 6 * (size_t)e.cap, 16 * (size_t)c->amp_cap, hipMemcpyDeviceToDevice, c->st));
     }
     // the cell list and pair list of the trial steps are stale now
     c->cells_fresh = false;
     c->skin_valid = false;
     c->have_density = false;
     c->time = t0;
-    HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(e.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MDPD_OK;
   };
   int rc = save();
   if (rc) return rc;
-  const int sizes[3] = {64, 128, 256};
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  float best = 1e30f;
-  mdpd_tuning bt = c->tune;
   const bool prof = c->profiling;
   c->profiling = false;
-  for (int bs : sizes)
+  // one candidate: MDPD_OK and its time in *ms, or a status (set_tuning
+  // refusals of a skin the boxes do not allow come back as MDPD_ERR_ARG)
+  auto trial = [&](const mdpd_tuning& t, float* ms) -> int {
+    int r;
+    if ((r = mdpd_set_tuning(c, &t))) return r;
+    if ((r = run_steps(c, 1))) return r;  // warm-up (graph capture)
+    if ((r = restore())) return r;
+    hipEventRecord(e0, c->st);
+    if ((r = run_steps(c, nsteps))) return r;
+    hipEventRecord(e1, c->st);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(ms, e0, e1);
+    if ((r = check_flags(c))) return r;
+    return restore();
+  };
+  const int sizes[3] = {64, 128, 256};
+  float best = 1e30f;
+  mdpd_tuning bt = c->tune;
+  bt.verlet_skin = 0.0f;
+  for (int bs : sizes) {
     for (int bf : sizes) {
       mdpd_tuning t = c->tune;
       t.block_scan = bs;
       t.block_force = bf;
-      if ((rc = mdpd_set_tuning(c, &t))) break;
-      HIPCHK(c, hipMemcpyAsync(d.d_time, &c->time, 4, hipMemcpyHostToDevice, c->st));
-      if ((rc = run_steps(c, 1))) break;  // warm-up (graph capture)
-      if ((rc = restore())) break;
-      hipEventRecord(e0, c->st);
-      if ((rc = run_steps(c, nsteps))) break;
-      hipEventRecord(e1, c->st);
-      hipEventSynchronize(e1);
+      if (flags & MDPD_TUNE_SKIN) t.verlet_skin = 0.0f;
       float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      if ((rc = check_flags(c))) break;
-      if ((rc = restore())) break;
+      if ((rc = trial(t, &ms))) break;
       if (ms < best) {
         best = ms;
         bt = t;
       }
     }
+    if (rc) break;
+  }
+  if (!rc && (flags & MDPD_TUNE_SKIN)) {
+    const float skins[3] = {0.1f, 0.2f, 0.3f};
+    for (float sk : skins) {
+      mdpd_tuning t = bt;
+      t.verlet_skin = sk;
+      float ms = 0.f;
+      const int r = trial(t, &ms);
+      if (r == MDPD_ERR_ARG) continue;  // the boxes hold too few cells of 1 + sk
+      if ((rc = r)) break;
+      if (ms < best) {
+        best = ms;
+        bt = t;
+      }
+    }
+  }
   c->profiling = prof;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -274,10 +308,14 @@ int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
   c->have_cells = c->cells_fresh = c->have_density = false;
   if (cache) {
     std::ofstream f(cache, std::ios::app);
-    f << key << " " << bt.block_scan << " " << bt.block_force << "\n";
+    f << key << " " << bt.block_scan << " " << bt.block_force << " " << bt.verlet_skin << "\n";
   }
   if (out) *out = c->tune;
   return MDPD_OK;
+}
+
+int mdpd_autotune(mdpd_ctx* c, uint32_t nsteps, mdpd_tuning* out) {
+  return mdpd_autotune_ex(c, nsteps, 0u, out);
 }
 
 }  // extern "C"

@@ -239,9 +239,12 @@ class Engine:
         raw = names.raw.split(b"\0")
         return {raw[k].decode(): float(ms[k]) for k in range(n.value)}
 
-    def autotune(self, nsteps=5):
+    def autotune(self, nsteps=5, skin=False):
+        """Fastest block sizes (and, with skin=True, neighbour-list skin) on
+        the current state; the state is left as it was."""
         t = MdpdTuning()
-        self._c(self.L.mdpd_autotune(self.h, int(nsteps), C.byref(t)))
+        flags = capi.MDPD_TUNE_SKIN if skin else 0
+        self._c(self.L.mdpd_autotune_ex(self.h, int(nsteps), flags, C.byref(t)))
         return t
 
 

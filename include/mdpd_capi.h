@@ -249,9 +249,17 @@ int mdpd_phase_times(mdpd_ctx* ctx, double* ms, int32_t cap, int32_t* n,
    HBM counters and confirms the bandwidth of the device. */
 int mdpd_bandwidth_probe(uint64_t bytes, int32_t reps, double* gbps);
 /* Times candidate launch choices on the current state (nsteps each) and
-   keeps the fastest; results are bit-identical across candidates except
-   for FP32 summation order.  out may be NULL. */
+   keeps the fastest; the state is restored after every candidate and the
+   results are bit-identical across candidates.  out may be NULL.  No
+   reference counterpart (project-only autotuner). */
 int mdpd_autotune(mdpd_ctx* ctx, uint32_t nsteps, mdpd_tuning* out);
+/* mdpd_autotune, and with MDPD_TUNE_SKIN also the neighbour-list bin: the
+   cell list rebuilt every step, or skins 0.1, 0.2, 0.3 (cells of 1 + skin,
+   where the boxes allow them).  A skin changes FP32 rounding only; the pair
+   sets stay exact. */
+#define MDPD_TUNE_SKIN 1u
+int mdpd_autotune_ex(mdpd_ctx* ctx, uint32_t nsteps, uint32_t flags,
+                     mdpd_tuning* out);
 
 /* ---- host-side parameter input (parameter.hpp:74-186,352-387) ---------- */
 /* Reads run_param.txt in `dir`; fills p with the reference coefficients of

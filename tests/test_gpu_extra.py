@@ -115,6 +115,36 @@ def test_autotuned_launches_are_bit_identical(engine_mod):
         assert np.array_equal(res[0][k], res[2][k]), k
 
 
+@pytest.mark.parametrize("box", [(10.0, 10.0, 8.0), (3.5, 6.0, 6.0)])
+def test_autotune_with_skin_choice(engine_mod, oracle, box):
+    """mdpd_autotune_ex with MDPD_TUNE_SKIN: a valid tuning (a skin the box
+    allows; the small box refuses 0.2 and 0.3 and those candidates are
+    skipped), the state left as it was, and exact pair sets afterwards."""
+    p = configs.groot_warren_params(box, pair_seed=2)
+    n = int(3 * box[0] * box[1] * box[2])
+    s = configs.random_fluid(n, box, 4)
+    e = engine_mod.Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"])
+    e.set_clock(9, p.pair_seed)
+    e.run(20)
+    before = e.download()
+    t = e.autotune(3, skin=True)
+    assert t.block_scan in (64, 128, 256) and t.block_force in (64, 128, 256)
+    allowed = [0.0] + [x for x in (0.1, 0.2, 0.3) if 3 * (1 + x) <= min(box) + 1e-9]
+    assert min(abs(t.verlet_skin - x) for x in allowed) < 1e-6, (t.verlet_skin, allowed)
+    assert e.get_clock() == 29
+    after = e.download()
+    for k in ("pos", "vel", "id", "prop"):
+        assert np.array_equal(before[k], after[k]), k
+    for _ in range(3):
+        e.run(4)
+        g = e.download()
+        gi, gj = e.pairs(1.0)
+        oi, oj = oracle.pairs(p, g["pos"], 1.0, brute=True)
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    e.close()
+
+
 def test_head_conversion_matches_oracle(engine_mod, oracle):
     box = (12.0, 12.0, 12.0)
     p = configs.warren_params(box, A=configs.MEMBRANE_A, B=25.0, rd=0.75, n_species=3, pair_seed=9)
