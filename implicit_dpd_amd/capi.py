@@ -168,6 +168,9 @@ PROTOTYPES = {
     "mdpd_observe": (C.c_int, [VP, P(MdpdObservables)]),
     "mdpd_observe_replicas": (C.c_int, [VP, P(MdpdObservables)]),
     "mdpd_density_profile": (C.c_int, [VP, C.c_int32, C.c_int32, _DP]),
+    "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
+    "mdpd_chain_histograms": (C.c_int, [VP, C.c_int32, C.c_double, _DP]),
+    "mdpd_count_map": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _DP]),
     "mdpd_run": (C.c_int, [VP, C.c_uint32]),
     "mdpd_sync": (C.c_int, [VP]),
     "mdpd_neighbor_pairs": (C.c_int, [VP, C.c_double, _U32P, _U32P, C.c_uint64, P(C.c_uint64)]),

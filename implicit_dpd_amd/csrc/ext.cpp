@@ -150,6 +150,84 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
   return MDPD_OK;
 }
 
+// ---- observables beyond the Observer (k_obs.hip)
+namespace {
+std::vector<double> sq_edges(int nb, double width) {
+  std::vector<double> e(nb + 1);
+  for (int k = 0; k <= nb; ++k) {
+    const double x = k * width;
+    e[k] = x * x;
+  }
+  return e;
+}
+}  // namespace
+
+int mdpd_rdf(mdpd_ctx* c, double rmax, int32_t nbins, double* out) {
+  if (!c || !out || !(rmax > 0.0 && rmax <= 1.0) || nbins < 1 || nbins > 1024)
+    return fail(c, MDPD_ERR_ARG, "rdf: 0 < rmax <= 1 and 1 <= nbins <= 1024");
+  if (!c->have_cells || !c->cells_fresh) return fail(c, MDPD_ERR_STATE, "no current cell list");
+  const std::vector<double> e2 = sq_edges(nbins, rmax / nbins);
+  const size_t cnt = (size_t)c->n_rep * nbins;
+  double* de = nullptr;
+  unsigned long long* h = nullptr;
+  HIPCHK(c, hipMalloc(&de, 8 * e2.size()));
+  HIPCHK(c, hipMalloc(&h, 8 * cnt));
+  HIPCHK(c, hipMemcpyAsync(de, e2.data(), 8 * e2.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(h, 0, 8 * cnt, c->st));
+  launch_rdf(c->d, de, nbins, rmax, h, c->st);
+  std::vector<unsigned long long> hv(cnt);
+  HIPCHK(c, hipMemcpyAsync(hv.data(), h, 8 * cnt, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  hipFree(de);
+  hipFree(h);
+  for (size_t k = 0; k < cnt; ++k) out[k] = (double)hv[k];
+  return MDPD_OK;
+}
+
+int mdpd_chain_histograms(mdpd_ctx* c, int32_t nbins, double bond_max, double* out) {
+  if (!c || !out || nbins < 1 || !(bond_max > 0.0))
+    return fail(c, MDPD_ERR_ARG, "chain histograms: nbins >= 1 and bond_max > 0");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "chain histograms between drift and kick");
+  const std::vector<double> eb2 = sq_edges(nbins, bond_max / nbins);
+  const std::vector<double> ee2 = sq_edges(nbins, 3.0 * bond_max / nbins);
+  const size_t cnt = (size_t)c->n_rep * 4 * nbins;
+  double* de = nullptr;
+  unsigned long long* h = nullptr;
+  HIPCHK(c, hipMalloc(&de, 16 * eb2.size()));
+  HIPCHK(c, hipMalloc(&h, 8 * cnt));
+  HIPCHK(c, hipMemcpyAsync(de, eb2.data(), 8 * eb2.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(de + eb2.size(), ee2.data(), 8 * ee2.size(), hipMemcpyHostToDevice,
+                           c->st));
+  HIPCHK(c, hipMemsetAsync(h, 0, 8 * cnt, c->st));
+  // the step keeps only the per-bead chain map current
+  launch_topo_from_chain(c->d, c->st);
+  launch_chain_hist(c->d, de, de + eb2.size(), nbins, h, c->st);
+  std::vector<unsigned long long> hv(cnt);
+  HIPCHK(c, hipMemcpyAsync(hv.data(), h, 8 * cnt, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  hipFree(de);
+  hipFree(h);
+  for (size_t k = 0; k < cnt; ++k) out[k] = (double)hv[k];
+  return MDPD_OK;
+}
+
+int mdpd_count_map(mdpd_ctx* c, int32_t gx, int32_t gy, int32_t gz, uint32_t type_mask,
+                   double* out) {
+  if (!c || !out || gx < 1 || gy < 1 || gz < 1 || (int64_t)gx * gy * gz > (1 << 26))
+    return fail(c, MDPD_ERR_ARG, "count map: 1 <= g per axis, at most 2^26 voxels");
+  const size_t cnt = (size_t)c->n_rep * gx * gy * gz;
+  unsigned int* h = nullptr;
+  HIPCHK(c, hipMalloc(&h, 4 * cnt));
+  HIPCHK(c, hipMemsetAsync(h, 0, 4 * cnt, c->st));
+  launch_count_map(c->d, gx, gy, gz, type_mask, h, c->st);
+  std::vector<unsigned int> hv(cnt);
+  HIPCHK(c, hipMemcpyAsync(hv.data(), h, 4 * cnt, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  hipFree(h);
+  for (size_t k = 0; k < cnt; ++k) out[k] = (double)hv[k];
+  return MDPD_OK;
+}
+
 // Times every (block_scan, block_force) pair on the current state for
 // `nsteps` steps and keeps the fastest.  The state is restored after every
 // candidate; per-bead sums run in a fixed order, so every candidate gives

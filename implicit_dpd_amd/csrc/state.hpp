@@ -136,6 +136,13 @@ void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, uint
                     double* out, hipStream_t st);
 void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist,
                             hipStream_t st);
+// observables of k_obs.hip: e2 / eb2 / ee2 are squared bin edges [nb + 1]
+void launch_rdf(const DevState& d, const double* e2, int nb, double rmax,
+                unsigned long long* hist, hipStream_t st);
+void launch_chain_hist(const DevState& d, const double* eb2, const double* ee2, int nb,
+                       unsigned long long* hist, hipStream_t st);
+void launch_count_map(const DevState& d, int g0, int g1, int g2, uint32_t type_mask,
+                      unsigned int* map, hipStream_t st);
 void launch_neighbor_cm(const DevState& d, const uint32_t* beads, uint32_t nb, double* cm_out,
                         hipStream_t st);
 void launch_pairs_list(const DevState& d, double cutoff, uint32_t* out_i, uint32_t* out_j,

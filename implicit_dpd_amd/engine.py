@@ -218,6 +218,26 @@ class Engine:
         self._c(self.L.mdpd_density_profile(self.h, axis, nbins, _d(out)))
         return out
 
+    def rdf(self, rmax, nbins):
+        """Ordered-pair counts per distance bin [n_rep][nbins] (mdpd_rdf)."""
+        out = np.zeros((self.n_rep, nbins))
+        self._c(self.L.mdpd_rdf(self.h, float(rmax), int(nbins), _d(out)))
+        return out
+
+    def chain_histograms(self, nbins, bond_max):
+        """[n_rep][4][nbins]: first bond, other bonds, bond-angle cosine,
+        end-to-end (mdpd_chain_histograms)."""
+        out = np.zeros((self.n_rep, 4, nbins))
+        self._c(self.L.mdpd_chain_histograms(self.h, int(nbins), float(bond_max), _d(out)))
+        return out
+
+    def count_map(self, g, type_mask=0xFFFFFFFF):
+        """Bead counts per voxel [n_rep][gz][gy][gx] (mdpd_count_map)."""
+        out = np.zeros((self.n_rep, g[2], g[1], g[0]))
+        self._c(self.L.mdpd_count_map(self.h, int(g[0]), int(g[1]), int(g[2]), int(type_mask),
+                                      _d(out)))
+        return out
+
     def pairs(self, cutoff=1.0):
         n = C.c_uint64()
         self._c(self.L.mdpd_neighbor_pairs(self.h, cutoff, None, None, 0, C.byref(n)))

@@ -209,6 +209,27 @@ int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
 int mdpd_observe(mdpd_ctx* ctx, mdpd_observables* out);
 /* observables of every replica of an ensemble context, out[n_rep] */
 int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
+/* Pair-distance histogram (radial distribution function; no reference
+   counterpart): ordered pairs i != j of the current positions with r in
+   [k dr, (k+1) dr), dr = rmax / nbins, decided on r^2 against the FP64
+   squared edges (k dr)^2; out[n_rep][nbins].  0 < rmax <= 1, nbins <= 1024;
+   needs the cell list of the current positions (after mdpd_run or
+   mdpd_rebuild_cells). */
+int mdpd_rdf(mdpd_ctx* ctx, double rmax, int32_t nbins, double* out);
+/* Chain geometry histograms, the quantities of
+   analysis/bond_angle/h2e.cpp:64-104, out[n_rep][4][nbins]: row 0 the first
+   bond |r1 - r0|, row 1 the other bonds (bins of bond_max / nbins, on the
+   squared length), row 2 the cosine of the angle between consecutive bonds
+   (bins of 2 / nbins over [-1, 1], decided without a square root), row 3
+   the end-to-end |r3 - r0| (bins of 3 bond_max / nbins). */
+int mdpd_chain_histograms(mdpd_ctx* ctx, int32_t nbins, double bond_max,
+                          double* out);
+/* 3-D count map of the beads whose type bit is set in type_mask
+   (analysis/trjanalysis/density_map.hpp:13-87, without its rotation into
+   the membrane's principal axes): voxels box / g per axis, index
+   (int)(x / edge) clamped to g - 1, out[n_rep][gz][gy][gx]. */
+int mdpd_count_map(mdpd_ctx* ctx, int32_t gx, int32_t gy, int32_t gz,
+                   uint32_t type_mask, double* out);
 /* bead-count profile along `axis` in nbins bins for every replica,
    out[n_rep][nbins] (counts per bin, FP64) */
 int mdpd_density_profile(mdpd_ctx* ctx, int32_t axis, int32_t nbins,

@@ -79,6 +79,32 @@ def test_full_size_pairs_density_force_match_oracle(engine_mod, oracle, name):
     _check_state(oracle, p, g, [x[0] for x in pairs], [x[1] for x in pairs], t)
 
 
+def test_membrane_full_size_observables(engine_mod, oracle):
+    """Pair-distance histogram, chain histograms (262,144 chains) and the
+    count map of the 1M-bead membrane after 20 steps, exact against numpy
+    on the downloaded positions (pairs from the oracle's cell list)."""
+    from obs_ref import chain_ref, count_ref, rdf_from_pairs
+    p, s = configs.membrane()
+    n = len(s["pos"])
+    e = engine_mod.Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(0, p.pair_seed)
+    e.run(20)
+    g = e.download()
+    rdf = e.rdf(0.75, 75)[0]
+    ch = e.chain_histograms(60, 1.5)[0]
+    cm = e.count_map((50, 50, 20), 0b011)[0]
+    e.close()
+    box = np.array(p.box[:])
+    i, j = oracle.pairs(p, g["pos"], 1.0)
+    assert np.array_equal(rdf, rdf_from_pairs(g["pos"], box, i, j, 0.75, 75))
+    ref = chain_ref(g["pos"], s["topo"], box, 60, 1.5)
+    for row in range(4):
+        assert np.array_equal(ch[row], ref[row]), row
+    assert np.array_equal(cm, count_ref(g["pos"], g["prop"], box, (50, 50, 20), 0b011))
+
+
 def test_ensemble_full_size_replicas_match_oracle(engine_mod, oracle):
     """The 64-replica ensemble (about 2.2M beads in one context, replica
     sizes U[4096, 65536], coefficients and seeds per replica) after 20
