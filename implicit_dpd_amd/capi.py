@@ -169,6 +169,8 @@ PROTOTYPES = {
     "mdpd_observe_replicas": (C.c_int, [VP, P(MdpdObservables)]),
     "mdpd_density_profile": (C.c_int, [VP, C.c_int32, C.c_int32, _DP]),
     "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
+    "mdpd_checkpoint_save": (C.c_int, [VP, C.c_char_p]),
+    "mdpd_checkpoint_load": (C.c_int, [VP, C.c_char_p]),
     "mdpd_chain_histograms": (C.c_int, [VP, C.c_int32, C.c_double, _DP]),
     "mdpd_count_map": (C.c_int, [VP, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _DP]),
     "mdpd_run": (C.c_int, [VP, C.c_uint32]),

@@ -150,6 +150,176 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
   return MDPD_OK;
 }
 
+// ---- binary checkpoint (exact restart)
+// Layout, little-endian: magic "MDPDCKP1", u32 version, n_rep, n, n_amp,
+// time, 0; u32 rep_off[n_rep + 1], pair_seed[n_rep], amp_num[n_rep],
+// sol_num[n_rep]; FP64 pos, vel, vel_buf, delta_sumr [3][n]; FP32 acc,
+// press [3][n]; u32 id, prop, amp_id, unit, orig, chain [n]; u32
+// topo[4 n_amp]; FP64 cf_bend[n_amp]; u32 amp_rep[n_amp].  Arrays are in the
+// device's storage order, so the next cell sort (ordered by previous slot)
+// and every FP32 sum after it repeat the uninterrupted run.
+namespace {
+constexpr char kCkpMagic[8] = {'M', 'D', 'P', 'D', 'C', 'K', 'P', '1'};
+constexpr uint32_t kCkpVersion = 1;
+}  // namespace
+
+int mdpd_checkpoint_save(mdpd_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, MDPD_ERR_ARG, "null argument");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "checkpoint between drift and kick");
+  const DevState& d = c->d;
+  const uint32_t n = d.n, na = d.n_amp, nr = (uint32_t)c->n_rep;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<double> f64(3 * (size_t)n);
+  std::vector<float> f32(3 * (size_t)n);
+  std::vector<uint32_t> u32((size_t)std::max(n, 4 * na));
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(c, MDPD_ERR_ARG, std::string("cannot open ") + path);
+  bool ok = true;
+  auto put = [&](const void* p, size_t bytes) { ok = ok && fwrite(p, 1, bytes, fp) == bytes; };
+  const uint32_t head[6] = {kCkpVersion, nr, n, na, c->time, 0u};
+  put(kCkpMagic, 8);
+  put(head, sizeof(head));
+  put(c->rep_off.data(), 4 * (size_t)(nr + 1));
+  for (uint32_t r = 0; r < nr; ++r) put(&c->R[r].pair_seed, 4);
+  put(c->amp_num_rep.data(), 4 * (size_t)nr);
+  put(c->sol_num_rep.data(), 4 * (size_t)nr);
+  int rc = MDPD_OK;
+  auto get3d = [&](const double* src) -> int {
+    for (int a = 0; a < 3; ++a)
+      HIPCHK(c, hipMemcpy(f64.data() + (size_t)a * n, src + (size_t)a * d.cap, 8 * (size_t)n,
+                          hipMemcpyDeviceToHost));
+    put(f64.data(), 8 * f64.size());
+    return MDPD_OK;
+  };
+  auto get3f = [&](const float* src) -> int {
+    for (int a = 0; a < 3; ++a)
+      HIPCHK(c, hipMemcpy(f32.data() + (size_t)a * n, src + (size_t)a * d.cap, 4 * (size_t)n,
+                          hipMemcpyDeviceToHost));
+    put(f32.data(), 4 * f32.size());
+    return MDPD_OK;
+  };
+  auto getu = [&](const uint32_t* src, size_t cnt) -> int {
+    HIPCHK(c, hipMemcpy(u32.data(), src, 4 * cnt, hipMemcpyDeviceToHost));
+    put(u32.data(), 4 * cnt);
+    return MDPD_OK;
+  };
+  if (!rc) rc = get3d(d.pos);
+  if (!rc) rc = get3d(d.vel);
+  if (!rc) rc = get3d(d.vbuf);
+  if (!rc) rc = get3d(d.delta);
+  if (!rc) rc = get3f(d.acc);
+  if (!rc) rc = get3f(d.press);
+  const uint32_t* const us[6] = {d.id, d.prop, d.amp_id, d.unit, d.orig, d.chain};
+  for (int q = 0; q < 6 && !rc; ++q) rc = getu(us[q], n);
+  if (!rc && na) {
+    // the step keeps only the per-bead chain map current
+    launch_topo_from_chain(d, c->st);
+    rc = getu(d.topo, 4 * (size_t)na);
+    if (!rc) {
+      std::vector<double> cf(na);
+      if (hipMemcpy(cf.data(), d.cf_bend, 8 * (size_t)na, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, MDPD_ERR_HIP, "checkpoint: copy of cf_bend failed");
+      put(cf.data(), 8 * (size_t)na);
+    }
+    if (!rc) rc = getu(d.amp_rep, na);
+  }
+  if (fclose(fp) != 0) ok = false;
+  if (rc) return rc;
+  if (!ok) return fail(c, MDPD_ERR_ARG, std::string("write failed: ") + path);
+  return MDPD_OK;
+}
+
+int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
+  if (c) c->skin_valid = false;
+  if (!c || !path) return fail(c, MDPD_ERR_ARG, "null argument");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(c, MDPD_ERR_ARG, std::string("cannot open ") + path);
+  bool ok = true;
+  auto get = [&](void* p, size_t bytes) { ok = ok && fread(p, 1, bytes, fp) == bytes; };
+  char magic[8];
+  uint32_t head[6];
+  get(magic, 8);
+  get(head, sizeof(head));
+  if (!ok || std::memcmp(magic, kCkpMagic, 8) != 0 || head[0] != kCkpVersion) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_PARSE, std::string("not a checkpoint of this format: ") + path);
+  }
+  const uint32_t nr = head[1], n = head[2], na = head[3];
+  if ((int)nr != c->n_rep || n > c->cap || na > c->amp_cap) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_ARG, "checkpoint does not fit this context (replicas or capacity)");
+  }
+  std::vector<uint32_t> roff(nr + 1), seed(nr), anum(nr), snum(nr);
+  get(roff.data(), 4 * (size_t)(nr + 1));
+  get(seed.data(), 4 * (size_t)nr);
+  get(anum.data(), 4 * (size_t)nr);
+  get(snum.data(), 4 * (size_t)nr);
+  if (!ok || (nr > 1 && roff != c->rep_off)) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_ARG, "checkpoint replica layout differs from this context");
+  }
+  std::vector<double> f64(12 * (size_t)n);
+  std::vector<float> f32(6 * (size_t)n);
+  std::vector<uint32_t> u32(6 * (size_t)n + 4 * (size_t)na + na);
+  std::vector<double> cf(na);
+  get(f64.data(), 8 * f64.size());
+  get(f32.data(), 4 * f32.size());
+  get(u32.data(), 4 * 6 * (size_t)n);
+  if (na) {
+    get(u32.data() + 6 * (size_t)n, 16 * (size_t)na);
+    get(cf.data(), 8 * (size_t)na);
+    get(u32.data() + 6 * (size_t)n + 4 * (size_t)na, 4 * (size_t)na);
+  }
+  fclose(fp);
+  if (!ok) return fail(c, MDPD_ERR_PARSE, std::string("truncated checkpoint: ") + path);
+  DevState& d = c->d;
+  double* const dd[4] = {d.pos, d.vel, d.vbuf, d.delta};
+  for (int q = 0; q < 4; ++q)
+    for (int a = 0; a < 3; ++a)
+      HIPCHK(c, hipMemcpy(dd[q] + (size_t)a * d.cap, f64.data() + ((size_t)q * 3 + a) * n,
+                          8 * (size_t)n, hipMemcpyHostToDevice));
+  float* const df[2] = {d.acc, d.press};
+  for (int q = 0; q < 2; ++q)
+    for (int a = 0; a < 3; ++a)
+      HIPCHK(c, hipMemcpy(df[q] + (size_t)a * d.cap, f32.data() + ((size_t)q * 3 + a) * n,
+                          4 * (size_t)n, hipMemcpyHostToDevice));
+  uint32_t* const du[6] = {d.id, d.prop, d.amp_id, d.unit, d.orig, d.chain};
+  for (int q = 0; q < 6; ++q)
+    HIPCHK(c, hipMemcpy(du[q], u32.data() + (size_t)q * n, 4 * (size_t)n, hipMemcpyHostToDevice));
+  if (na) {
+    HIPCHK(c, hipMemcpy(d.topo, u32.data() + 6 * (size_t)n, 16 * (size_t)na, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.cf_bend, cf.data(), 8 * (size_t)na, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.amp_rep, u32.data() + 6 * (size_t)n + 4 * (size_t)na, 4 * (size_t)na,
+                        hipMemcpyHostToDevice));
+  }
+  // host mirrors: bead and chain counts, seeds, clock
+  c->amp_num_rep = anum;
+  c->sol_num_rep = snum;
+  c->amp_off.assign(nr + 1, 0);
+  for (uint32_t r = 0; r < nr; ++r) c->amp_off[r + 1] = c->amp_off[r] + anum[r];
+  for (uint32_t r = 0; r < nr; ++r) {
+    c->R[r].pair_seed = seed[r];
+    c->P[r].pair_seed = seed[r];
+  }
+  if (nr == 1) {
+    c->rep_off[1] = n;
+    c->R[0].n_beads = n;
+  }
+  HIPCHK(c, hipMemcpy(d.reps, c->R.data(), sizeof(Replica) * nr, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d.rep_off, c->rep_off.data(), 4 * (size_t)(nr + 1), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d.amp_off, c->amp_off.data(), 4 * (size_t)(nr + 1), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_amp_num, anum.data(), 4 * (size_t)nr, hipMemcpyHostToDevice));
+  invalidate_graphs(c);
+  d.n = n;
+  d.n_amp = na;
+  c->n_amp = na;
+  c->time = head[4];
+  HIPCHK(c, hipMemcpy(d.d_time, &c->time, 4, hipMemcpyHostToDevice));
+  c->drifted = false;
+  c->have_cells = c->cells_fresh = c->have_density = false;
+  return MDPD_OK;
+}
+
 // ---- observables beyond the Observer (k_obs.hip)
 namespace {
 std::vector<double> sq_edges(int nb, double width) {

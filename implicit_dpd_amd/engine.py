@@ -218,6 +218,12 @@ class Engine:
         self._c(self.L.mdpd_density_profile(self.h, axis, nbins, _d(out)))
         return out
 
+    def checkpoint_save(self, path):
+        self._c(self.L.mdpd_checkpoint_save(self.h, str(path).encode()))
+
+    def checkpoint_load(self, path):
+        self._c(self.L.mdpd_checkpoint_load(self.h, str(path).encode()))
+
     def rdf(self, rmax, nbins):
         """Ordered-pair counts per distance bin [n_rep][nbins] (mdpd_rdf)."""
         out = np.zeros((self.n_rep, nbins))

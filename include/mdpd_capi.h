@@ -209,6 +209,16 @@ int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
 int mdpd_observe(mdpd_ctx* ctx, mdpd_observables* out);
 /* observables of every replica of an ensemble context, out[n_rep] */
 int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
+/* Binary checkpoint of the whole evolving state in the device's storage
+   order (positions, velocities, vel_buf, delta_sumr, accelerations, virials,
+   ids and types, chain map and topology, clock and pair seeds).  Loading it
+   into a context created with the same parameters and capacity continues
+   the run bit for bit (the every-step cell list; in skin mode the first
+   step after a load rebuilds, so FP32 rounding may differ).  The
+   reference restarts from ASCII (restart.py, fin_config.xyz); this is the
+   exact-restart sidecar. */
+int mdpd_checkpoint_save(mdpd_ctx* ctx, const char* path);
+int mdpd_checkpoint_load(mdpd_ctx* ctx, const char* path);
 /* Pair-distance histogram (radial distribution function; no reference
    counterpart): ordered pairs i != j of the current positions with r in
    [k dr, (k+1) dr), dr = rmax / nbins, decided on r^2 against the FP64
