@@ -142,9 +142,12 @@ def test_ensemble_full_size_replicas_match_oracle(engine_mod, oracle):
         _check_state(oracle, p, gr, [x[0] for x in sel], [x[1] for x in sel], t, base=b0)
 
 
-def _run_observe(sim, warm, blocks, every):
+def _run_observe(sim, warm, blocks, every, sample=None):
+    """Mean temperature and pressure over `blocks` observations `every`
+    steps apart after `warm` steps; `sample(sim)` is called at each
+    observation too and its results are returned stacked."""
     sim.run(warm)
-    T, P = [], []
+    T, P, S = [], [], []
     for _ in range(blocks):
         sim.run(every)
         o = sim.observe()
@@ -152,23 +155,73 @@ def _run_observe(sim, warm, blocks, every):
         pr = o["pressure"] if isinstance(o, dict) else o.pressure[:]
         T.append(float(np.mean(kt[:3])))
         P.append(float(np.mean(pr[:3])))
-    return np.mean(T), np.mean(P)
+        if sample is not None:
+            S.append(sample(sim))
+    return np.mean(T), np.mean(P), (np.array(S) if S else None)
+
+
+def _agree(a, b, z=5.0, floor=0.0):
+    """Two stacks of sampled histograms agree bin by bin within z combined
+    standard errors of their means (plus an absolute floor)."""
+    ma, mb = a.mean(axis=0), b.mean(axis=0)
+    se = np.sqrt(a.var(axis=0, ddof=1) / len(a) + b.var(axis=0, ddof=1) / len(b))
+    bad = np.abs(ma - mb) > z * se + floor
+    return not bad.any(), (np.abs(ma - mb) / np.maximum(se, 1e-12)).max()
 
 
 def test_groot_warren_24k_equation_of_state(engine_mod, oracle):
+    """Long run on both sides from the same start: temperature, pressure and
+    the pair-distance histogram agree within statistical error."""
+    from obs_ref import rdf_from_pairs
     p, s = configs.groot_warren_fluid()
     n = len(s["pos"])
+    box = np.array(p.box[:])
+    rmax, nb = 0.96, 48  # the top edge stays inside the oracle's r < 1 pair list
     e = engine_mod.Engine(p, capacity=n)
     e.upload(s["pos"], s["vel"], s["ids"], s["prop"])
     e.set_clock(0, p.pair_seed)
-    Tg, Pg = _run_observe(e, 400, 12, 50)
+    Tg, Pg, Hg = _run_observe(e, 400, 12, 50, lambda sim: sim.rdf(rmax, nb)[0])
     e.close()
     o = oracle.System(p, n)
     o.set(s["pos"], s["vel"], s["ids"], s["prop"])
     o.set_clock(0)
-    To, Po = _run_observe(o, 400, 12, 50)
+
+    def orc_rdf(sim):
+        pos = sim.get()["pos"]
+        i, j = oracle.pairs(p, pos, 1.0)
+        return rdf_from_pairs(pos, box, i, j, rmax, nb)
+
+    To, Po, Ho = _run_observe(o, 400, 12, 50, orc_rdf)
     rho = n / (20.0 ** 3)
     alpha = (Pg - rho) / (25.0 * rho * rho)
     assert abs(Tg - 1.0) < 0.015, Tg
     assert abs(Pg - Po) < 0.015 * Po, (Pg, Po)
     assert 0.085 < alpha < 0.105, (Pg, alpha)
+    ok, zmax = _agree(Hg, Ho, floor=2.0)
+    assert ok, zmax
+
+
+def test_chain_system_long_run_histograms(engine_mod, oracle):
+    """4096 H1T3 chains (16,384 beads) of the membrane model: bond-length,
+    bond-angle and end-to-end histograms of independent long runs on the
+    GPU and on the oracle agree within statistical error."""
+    from obs_ref import chain_ref
+    p, s = configs.membrane(n_chain=4096, box=(25.0, 25.0, 10.0), conv_p=0.0)
+    n = len(s["pos"])
+    box = np.array(p.box[:])
+    nb, bmax = 40, 1.2
+    e = engine_mod.Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(0, p.pair_seed)
+    _, Pg, Hg = _run_observe(e, 300, 10, 40, lambda sim: sim.chain_histograms(nb, bmax)[0])
+    e.close()
+    o = oracle.System(p, n, len(s["topo"]) // 4 + 1)
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    o.set_topology(s["topo"])
+    o.set_clock(0)
+    _, Po, Ho = _run_observe(o, 300, 10, 40,
+                             lambda sim: chain_ref(sim.get()["pos"], s["topo"], box, nb, bmax))
+    for row in range(4):
+        ok, zmax = _agree(Hg[:, row], Ho[:, row], floor=2.0)
+        assert ok, (row, zmax)
