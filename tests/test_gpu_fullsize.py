@@ -225,3 +225,32 @@ def test_chain_system_long_run_histograms(engine_mod, oracle):
     for row in range(4):
         ok, zmax = _agree(Hg[:, row], Ho[:, row], floor=2.0)
         assert ok, (row, zmax)
+
+
+def test_slab_long_run_density_profile(engine_mod, oracle):
+    """A 24,576-bead MDPD liquid slab (A = -40, B = 25, rd = 0.75) in a
+    16 x 16 x 32 box: the z density profile of independent long runs on the
+    GPU and on the oracle agrees within statistical error."""
+    p, s = configs.mdpd_slab(n=24576, box=(16.0, 16.0, 32.0))
+    n = len(s["pos"])
+    nb = 32
+    e = engine_mod.Engine(p, capacity=n)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"])
+    e.set_clock(0, p.pair_seed)
+    _, _, Hg = _run_observe(e, 300, 10, 30, lambda sim: sim.density_profile(2, nb)[0])
+    e.close()
+    o = oracle.System(p, n)
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"])
+    o.set_clock(0)
+
+    def orc_profile(sim):
+        z = sim.get()["pos"][:, 2]
+        b = np.clip(np.floor(z * (1.0 / 32.0) * nb).astype(int), 0, nb - 1)
+        return np.bincount(b, minlength=nb).astype(float)
+
+    _, _, Ho = _run_observe(o, 300, 10, 30, orc_profile)
+    ok, zmax = _agree(Hg, Ho, floor=2.0)
+    assert ok, zmax
+    # a liquid slab: the middle is dense, the edges of the box nearly empty
+    mid = Hg.mean(axis=0)[nb // 4:3 * nb // 4].mean()
+    assert mid > 20 * max(Hg.mean(axis=0)[0], 1.0)
