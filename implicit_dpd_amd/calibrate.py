@@ -9,7 +9,9 @@ P_zz = 0 (the coexistence condition).
   started in the middle half at density 6), run on the GPU; it measures the
   slab's bulk density from the z density profile and P_zz, averaged over the
   second half of the run.
-- The *sweep* finds A by regula falsi on plain runs alone.
+- The *sweep* finds A by regula falsi on plain runs alone; the *grid
+  sweep* (``--grid``) steps A by 1 from the dilute end of the range until a
+  run is within 1% or passes the target, then bisects the last interval.
 - The *surrogate* is trained on ensemble runs (64 replicas of U[4k, 64k]
   beads per launch with A ~ U[-60,-20], B ~ U[10,40], rd ~ U[0.65,0.85],
   kBT = 1) and proposes A by scanning the network; each proposal is verified
@@ -164,6 +166,42 @@ def sweep(plain, target=TARGET_RHO, lo=A_RANGE[0], hi=A_RANGE[1], max_runs=12):
     return None, runs
 
 
+def grid_sweep(plain, target=TARGET_RHO, lo=A_RANGE[0], hi=A_RANGE[1], step=1.0,
+               max_refine=8):
+    """The plain parameter sweep: A on a grid of `step` from the dilute end
+    (hi) towards lo until a run is within TOL of the target or passes it,
+    then bisection between the last two grid points.  Returns (A, runs); A
+    is None when the grid never reaches the target."""
+    runs = []
+    prev = None
+    n = int(round((hi - lo) / step))
+    for k in range(n + 1):
+        a = hi - k * step
+        rho, _ = plain(a)
+        runs.append((a, rho))
+        if abs(rho - target) <= TOL * target:
+            return a, runs
+        if rho > target:
+            break
+        prev = (a, rho)
+    else:
+        return None, runs
+    if prev is None:
+        return None, runs  # already past the target at the dilute end
+    (a_hi, _), (a_lo, _) = prev, runs[-1]
+    for _ in range(max_refine):
+        a = 0.5 * (a_hi + a_lo)
+        rho, _ = plain(a)
+        runs.append((a, rho))
+        if abs(rho - target) <= TOL * target:
+            return a, runs
+        if rho > target:
+            a_lo = a
+        else:
+            a_hi = a
+    return None, runs
+
+
 def train_surrogate(X, Y, epochs=300, seed=3):
     xs, ys = Scaler(X), Scaler(Y)
     net = Surrogate(max_batch=4096, seed=seed)
@@ -228,12 +266,16 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--more-targets", action="store_true",
                     help="also calibrate six further (rho, B, rd) targets")
+    ap.add_argument("--grid", action="store_true",
+                    help="also run the plain grid sweep (A in steps of 1 from -20, then "
+                         "bisection)")
     ap.add_argument("--save-samples", default=None, help="write the ensemble samples (.npz)")
     ap.add_argument("--load-samples", default=None,
                     help="reuse ensemble samples written by --save-samples")
     args = ap.parse_args(argv)
 
-    secs = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0}
+    secs = {"samples": 0.0, "train": 0.0, "surrogate_runs": 0.0, "sweep_runs": 0.0,
+            "grid_runs": 0.0}
 
     def plain(key, B, rd):
         def run(a):
@@ -263,22 +305,29 @@ def main(argv=None):
         a_s, runs_s = surrogate_calibrate(net, xs, ys, plain("surrogate_runs", B, rd),
                                           target=rho_t, B=B, rd=rd)
         a_p, runs_p = sweep(plain("sweep_runs", B, rd), target=rho_t)
+        a_g, runs_g = (grid_sweep(plain("grid_runs", B, rd), target=rho_t) if args.grid
+                       else (None, []))
         results.append({
             "target": {"rho": rho_t, "tol": TOL, "B": B, "rd": rd, "kBT": 1.0},
             "surrogate": {"A": a_s, "plain_runs": len(runs_s),
                           "runs": [{"A": a, "rho": r, "rho_pred": q} for a, r, q in runs_s]},
             "sweep": {"A": a_p, "plain_runs": len(runs_p),
-                      "runs": [{"A": a, "rho": r} for a, r in runs_p]}})
+                      "runs": [{"A": a, "rho": r} for a, r in runs_p]},
+            "grid": {"A": a_g, "plain_runs": len(runs_g),
+                     "runs": [{"A": a, "rho": r} for a, r in runs_g]}})
     net.close()
     n_s = sum(r["surrogate"]["plain_runs"] for r in results)
     n_p = sum(r["sweep"]["plain_runs"] for r in results)
+    n_g = sum(r["grid"]["plain_runs"] for r in results)
     out = {
         "samples": int(len(X)),
         "train_loss_first_last": [float(hist[0]), float(hist[-1])],
         "targets": results,
-        "plain_runs": {"surrogate": n_s, "sweep": n_p, "ratio": round(n_p / max(n_s, 1), 3)},
+        "plain_runs": {"surrogate": n_s, "sweep": n_p, "ratio": round(n_p / max(n_s, 1), 3),
+                       "grid": n_g, "grid_ratio": round(n_g / max(n_s, 1), 3)},
         "hits": {"surrogate": sum(r["surrogate"]["A"] is not None for r in results),
-                 "sweep": sum(r["sweep"]["A"] is not None for r in results)},
+                 "sweep": sum(r["sweep"]["A"] is not None for r in results),
+                 "grid": sum(r["grid"]["A"] is not None for r in results)},
         "seconds": {k: round(v, 2) for k, v in secs.items()},
     }
     print(json.dumps(out), flush=True)
