@@ -179,6 +179,54 @@ def workload(name, chains):
     raise SystemExit(f"unknown workload {name}")
 
 
+def config_gen_bench(args, Engine):
+    """Initial configuration (config_maker's flat mode): --chains H1T3
+    amphiphiles as a bilayer in the 100x100x40 membrane box with Maxwell-
+    Boltzmann velocities, generated on the device straight into an engine
+    context (mdpd_make_config), K times after W warm-up calls; the oracle's
+    sequential restatement of config_maker times the same spec on one host
+    core.  Per bead the device writes pos, vel, vel_buf (3 x 24 B), the four
+    ids (16 B), delta_sumr and acc (24 + 12 B) and chain/slot maps (8 B), and
+    reads the velocities twice for the drift and temperature sums (48 B)."""
+    from implicit_dpd_amd import config_maker as cm
+    from implicit_dpd_amd import configs
+    box = (100.0, 100.0, 40.0)
+    spec = cm.spec("flat", box, args.chains, seed=21)
+    n = cm.config_count(spec)
+    p, _ = configs.membrane(n_chain=1, box=box)
+    e = Engine(p, capacity=n)
+    for _ in range(max(args.warmup, 1)):
+        e.make_config(spec)
+    steps = max(args.steps, 1)
+    t = time.perf_counter()
+    for _ in range(steps):
+        kt = e.make_config(spec)
+    wall = (time.perf_counter() - t) / steps
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        t = time.perf_counter()
+        rc, _ = po.make_config(spec)
+        dt = time.perf_counter() - t
+        assert rc == 0
+        cpu = {"value": round(n / dt, 1), "unit": "beads/s", "cores": 1, "kind": "port",
+               "sample": f"oracle restatement of config_maker (sequential), the same {n}-bead spec, "
+                         f"{dt:.2f} s"}
+    bytes_per_bead = 3 * 24 + 16 + 36 + 8 + 48
+    print(json.dumps({
+        "metric": "initial-configuration beads/s (config_maker flat bilayer, generated on the device "
+                  "into the engine context)",
+        "value": round(n / wall, 1), "unit": "beads/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_call": round(wall * 1e3, 4), "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"flat bilayer, {args.chains} H1T3 amphiphiles ({n} beads), box "
+                               "100x100x40, T=1, seed 21", "kin_temp": kt},
+        "algorithmic_bytes_per_bead": bytes_per_bead,
+        "achieved_gbs_wall": round(n * bytes_per_bead / wall / 1e9, 1),
+        "cpu_baseline": cpu}), flush=True)
+    e.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +242,9 @@ def main():
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--skin", type=float, default=0.0,
                     help="neighbour-list skin (0: cell list rebuilt every step)")
+    ap.add_argument("--config-gen", action="store_true",
+                    help="time the initial-configuration generator instead (flat bilayer of "
+                         "--chains H1T3 amphiphiles in the membrane box), next to the oracle")
     ap.add_argument("--calib-bytes", type=float, default=0.0,
                     help="only run the copy-kernel probe of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -209,6 +260,10 @@ def main():
         g = bandwidth_probe(int(args.calib_bytes), 10)
         print(json.dumps({"copy_probe_bytes": int(args.calib_bytes), "reps": 11,
                           "gbps": g}), flush=True)
+        return
+
+    if args.config_gen:
+        config_gen_bench(args, Engine)
         return
 
     w = workload(args.workload, args.chains)

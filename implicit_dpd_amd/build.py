@@ -27,7 +27,8 @@ FLAGS = [
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["k_step.hip", "k_pair.hip", "k_obs.hip", "k_mlp.hip", "engine.cpp", "params.cpp", "ext.cpp", "mlp.cpp"]
+SOURCES = ["k_step.hip", "k_pair.hip", "k_obs.hip", "k_mlp.hip", "k_config.hip", "engine.cpp", "params.cpp",
+           "ext.cpp", "mlp.cpp", "config.cpp"]
 
 
 def _compile(src):

@@ -22,6 +22,10 @@ MDPD_ERR_PARSE = 7
 MDPD_ERR_STATE = 8
 MDPD_ERR_NO_DEVICE = 9
 MDPD_ERR_UNSUPPORTED = 10
+MDPD_ERR_CONFIG = 11
+
+# initial-configuration modes (config_maker.cpp GenConfig)
+CFG_FLAT, CFG_ASYM_FLAT, CFG_SPHERE, CFG_CYLIND, CFG_RANDOM = 0, 1, 2, 3, 4
 
 HYPHIL, HYPHOB, SOLVENT = 0, 1, 2
 FPDPD_STRIDE = 232
@@ -131,6 +135,28 @@ class MdpdRunInput(C.Structure):
     ]
 
 
+class MdpdConfigSpec(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("with_solvent", C.c_int32),
+        ("box", C.c_double * 3),
+        ("temperature", C.c_double),
+        ("amp_num", C.c_uint32),
+        ("head_unit", C.c_uint32),
+        ("tail_unit", C.c_uint32),
+        ("sol_num", C.c_uint32),
+        ("bond_leng", C.c_double),
+        ("lip_len", C.c_double),
+        ("sph_rad", C.c_double),
+        ("upper_the", C.c_double),
+        ("cyl_l", C.c_double),
+        ("cyl_r", C.c_double),
+        ("in_out_rat", C.c_double),
+        ("seed", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
 P = C.POINTER
 VP = C.c_void_p
 _U32P = P(C.c_uint32)
@@ -184,6 +210,12 @@ PROTOTYPES = {
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
     "mdpd_run_timed": (C.c_int, [VP, C.c_uint32, _DP]),
+    # initial configurations
+    "mdpd_read_config_param": (C.c_int, [C.c_char_p, P(MdpdConfigSpec)]),
+    "mdpd_config_count": (C.c_int, [P(MdpdConfigSpec), _U32P]),
+    "mdpd_make_config_host": (C.c_int, [P(MdpdConfigSpec), C.c_uint32, _DP, _DP, _U32P, _U32P,
+                                        _U32P, _U32P, _DP]),
+    "mdpd_make_config": (C.c_int, [VP, P(MdpdConfigSpec), _DP]),
     "mdpd_skin_stats": (C.c_int, [VP, P(C.c_uint64), P(C.c_uint64)]),
     # equation-of-state surrogate
     "mdpd_mlp_num_params": (C.c_int, [_U32P]),

@@ -126,6 +126,14 @@ class Engine:
         self._c(self.L.mdpd_set_topology(self.h, _u(topo) if len(topo) else None,
                                          len(topo) // 4, _u(core), len(core_amp_id)))
 
+    def make_config(self, spec):
+        """Generates the configuration of `spec` (config_maker.py) on the
+        device as this context's state, 4-bead amphiphiles as its chains;
+        returns the kinetic temperature."""
+        kt = C.c_double()
+        self._c(self.L.mdpd_make_config(self.h, C.byref(spec), C.byref(kt)))
+        return kt.value
+
     def set_clock(self, time, pair_seed):
         self._c(self.L.mdpd_set_clock(self.h, int(time), int(pair_seed)))
 

@@ -64,7 +64,8 @@ enum {
   MDPD_ERR_PARSE = 7,         /* run_param.txt / xyz parse error */
   MDPD_ERR_STATE = 8,         /* call out of order (e.g. force before density) */
   MDPD_ERR_NO_DEVICE = 9,     /* no HIP device */
-  MDPD_ERR_UNSUPPORTED = 10   /* option not available for this context */
+  MDPD_ERR_UNSUPPORTED = 10,  /* option not available for this context */
+  MDPD_ERR_CONFIG = 11        /* a generated configuration fails config_maker's checks */
 };
 
 /* bead types, parameter.hpp:211-217 */
@@ -308,6 +309,70 @@ typedef struct mdpd_run_input {
   uint32_t core_amp_id[64];
 } mdpd_run_input;
 int mdpd_read_run_param(const char* dir, mdpd_params* p, mdpd_run_input* in);
+
+/* ---- initial configurations (src/config_maker.cpp) ---------------------
+   The input of the reference's ConfigMaker (config_maker.cpp:14-38, tags
+   read by MatchingTagValues :386-418).  Modes (GenConfig, :342-384):
+     FLAT       bilayer normal to y, leaflets alternating amphiphile by
+                amphiphile (MakeFlatSheet :103-138); solvent in the lower
+                half of y (MakeCuboidSolvent :282-293)
+     ASYM_FLAT  as FLAT, but each block of 100 amphiphiles puts its first
+                (int)(100 in_out_rat) in the upper leaflet (:113-137)
+     SPHERE     vesicle of rings of spacing lip_len, outer leaflet then
+                inner (MakeSphSheet/MakeSphLine :139-213); solvent inside
+                (MakeSphericalSolvent :295-321)
+     CYLIND     cylinder along z (MakeCylindSheet :215-262); solvent inside
+     RANDOM     random head positions, directions in the positive octant
+                (MakeRandomConfig :264-280)
+   Velocities are N(0, Temperature) per component with the centre-of-mass
+   drift removed, vel_buf = vel, acc = 0 (GenVeloc :60-72).
+   Random draws: the reference seeds MT19937 with time(NULL) and draws in
+   bead order; here every amphiphile, solvent bead and velocity has its own
+   keyed Saru stream (bead or amphiphile index, stream tag, seed), so the
+   generator runs one thread per bead and a given seed always gives the
+   same configuration. */
+enum { MDPD_CFG_FLAT = 0, MDPD_CFG_ASYM_FLAT = 1, MDPD_CFG_SPHERE = 2,
+       MDPD_CFG_CYLIND = 3, MDPD_CFG_RANDOM = 4 };
+typedef struct mdpd_config_spec {
+  int32_t mode;             /* MDPD_CFG_* */
+  int32_t with_solvent;     /* tag with_solvent present */
+  double box[3];            /* box_leng */
+  double temperature;       /* Temperature */
+  uint32_t amp_num, head_unit, tail_unit, sol_num;
+  double bond_leng, lip_len;
+  double sph_rad, upper_the;  /* SPHERE; upper_the in degrees */
+  double cyl_l, cyl_r;        /* CYLIND */
+  double in_out_rat;          /* ASYM_FLAT */
+  uint32_t seed;              /* key of every draw */
+  uint32_t pad;
+} mdpd_config_spec;
+/* Reads config_param.txt in `dir` (ConfigMaker::LoadParam, :455-467):
+   required tags Temperature, box_leng, amp_num, mode, head_unit, tail_unit,
+   bond_leng, lip_len, plus sph_rad/upper_the, cyl_l/cyl_r, in_out_rat,
+   sol_num by mode; values through std::stof / std::stoi like the
+   reference.  seed is left 0. */
+int mdpd_read_config_param(const char* dir, mdpd_config_spec* s);
+/* beads the spec makes: amp_num (head_unit + tail_unit) + sol_num if
+   with_solvent (GenParticles :469-477) */
+int mdpd_config_count(const mdpd_config_spec* s, uint32_t* n_beads);
+/* Generates the configuration on the device into host arrays of n =
+   mdpd_config_count beads, in the reference's bead order (amphiphile a is
+   beads a*all_unit .. a*all_unit + all_unit-1, solvent follows; id = bead
+   index).  pos/vel [n][3].  Errors follow config_maker's exits: sphere
+   radius too large, too many amphiphiles for the rings, a bead outside
+   [0, L], |kinetic temperature - Temperature| >= 1e-2
+   (CheckParticleConfigIsValid :420-452), with the message in
+   mdpd_global_error(); kin_temp (may be NULL) gets sum v.v / (3 n). */
+int mdpd_make_config_host(const mdpd_config_spec* s, uint32_t n, double* pos,
+                          double* vel, uint32_t* id, uint32_t* prop,
+                          uint32_t* amp_id, uint32_t* unit, double* kin_temp);
+/* The same configuration generated straight into a single-system context
+   (replaces config_maker + init_config.xyz + mdpd_upload_fpdpd): the beads
+   become the context's state and, for 4-bead amphiphiles (head_unit +
+   tail_unit == 4, the engine's chain model), the chains its topology with
+   no core amphiphiles (mdpd_set_topology may follow).  Other chain lengths
+   are refused with MDPD_ERR_UNSUPPORTED. */
+int mdpd_make_config(mdpd_ctx* ctx, const mdpd_config_spec* s, double* kin_temp);
 
 /* ---- equation-of-state surrogate (no reference counterpart) ------------
    A 4 -> 128 -> 128 -> 2 perceptron with SiLU hidden layers that maps

@@ -900,3 +900,252 @@ int oracle_num_threads(void) {
   return 1;
 #endif
 }
+
+/* ======================================================================
+ * Initial configurations -- src/config_maker.cpp, walked sequentially as
+ * the reference does: GenConfig (:342-384), MakeFlatSheet (:103-138, flap
+ * toggled amphiphile by amphiphile), MakeSphSheet / MakeLineForEachTheta /
+ * MakeSphLine (:139-213), MakeCylindSheet / MakeLineForEachAxis
+ * (:215-262, including their missing guard: a pass entered when full lays
+ * one amphiphile past the end and the run fails), MakeRandomConfig
+ * (:264-280), MakeCuboidSolvent / MakeSphericalSolvent (:282-321), GenVeloc
+ * with RemoveCMDrift (:51-72, plain sequential sums) and
+ * CheckParticleConfigIsValid (:420-452).  The MT19937 draws become keyed
+ * Saru streams, one per amphiphile / solvent bead / velocity:
+ * Saru(index, tag, seed), a uniform from the top 26 bits of two draws.
+ * ====================================================================== */
+#define CFG_TAG_AMP 0x414D5048u
+#define CFG_TAG_SOL 0x534F4C56u
+#define CFG_TAG_VEL 0x56454C4Fu
+#define CFG_SOL_MAX_ATTEMPTS (1u << 16)
+
+static inline double o_cfg_uni(o_saru* s, int open) {
+  const uint32_t a = o_saru_u32(s) >> 6, b = o_saru_u32(s) >> 6;
+  const double k = (double)a * 67108864.0 + (double)b;
+  return (open ? k + 0.5 : k) * (1.0 / 4503599627370496.0);
+}
+
+typedef struct {
+  const mdpd_config_spec* s;
+  uint32_t all_unit, end_idx, idx, n;
+  double *pos;
+  uint32_t *id, *prop, *amp_id, *unit;
+} o_cfg;
+
+static void o_cfg_pbc(const double* box, double* p) {
+  for (int a = 0; a < 3; ++a) p[a] -= floor(p[a] / box[a]) * box[a];
+}
+
+/* SetAmphilPartPos :79-94; beads past the array (the axis passes'
+   overshoot) are counted, not written */
+static void o_cfg_amph(o_cfg* c, const double base[3], const double nv[3]) {
+  for (uint32_t u = 0; u < c->all_unit; ++u) {
+    double p[3];
+    for (int a = 0; a < 3; ++a) p[a] = base[a] + (c->s->bond_leng * u) * nv[a];
+    o_cfg_pbc(c->s->box, p);
+    const uint32_t i = c->idx;
+    if (i < c->n) {
+      memcpy(c->pos + 3 * (size_t)i, p, sizeof(p));
+      c->id[i] = i;
+      c->unit[i] = u;
+      c->amp_id[i] = i / c->all_unit;
+      c->prop[i] = u < c->s->head_unit ? MDPD_HYPHIL : MDPD_HYPHOB;
+    }
+    c->idx++;
+  }
+}
+
+/* MakeSphLine :139-161; returns 0 once the amphiphiles are all laid */
+static int o_cfg_sph_line(o_cfg* c, double the, const double center[3], double rad, double sign) {
+  const double q_thick = 0.5 * c->all_unit * c->s->bond_leng;
+  const double prj_rad = rad * sin(the);
+  double d_phi = c->s->lip_len / prj_rad;
+  const int32_t phi_elem = (int32_t)(2.0 * M_PI / d_phi);
+  d_phi = 2.0 * M_PI / phi_elem;
+  for (int32_t i = 0; i < phi_elem; i++) {
+    const double phi = d_phi * i;
+    const double nv[3] = {sign * sin(the) * cos(phi), sign * sin(the) * sin(phi), sign * cos(the)};
+    double base[3];
+    for (int a = 0; a < 3; ++a) base[a] = sign * nv[a] * (rad - sign * q_thick) + center[a];
+    o_cfg_amph(c, base, nv);
+    if (c->idx >= c->end_idx) return 0;
+  }
+  return 1;
+}
+
+static void o_cfg_theta_pass(o_cfg* c, int32_t elem, double d_the, double offset, double rad,
+                             double sign) {
+  if (c->idx >= c->end_idx) return;
+  const double cent[3] = {0.5 * c->s->box[0], 0.5 * c->s->box[1], 0.5 * c->s->box[2]};
+  for (int32_t i = 0; i < elem; i++)
+    if (!o_cfg_sph_line(c, d_the * (i + offset), cent, rad, sign)) return;
+}
+
+static void o_cfg_axis_pass(o_cfg* c, int32_t elem, double rad, double sign) {
+  const double h_pi = acos(0.0);
+  double cent[3] = {0.5 * c->s->box[0], 0.5 * c->s->box[1], 0.5 * c->s->box[2]};
+  for (int32_t i = 0; i < elem; i++) {
+    cent[2] = i * c->s->lip_len;
+    if (!o_cfg_sph_line(c, h_pi, cent, rad, sign)) return;
+  }
+}
+
+/* the count check of the ring passes; elem from a double as the reference's
+   static_cast<S32> (finite, in range by the checks below) */
+static int32_t o_cfg_elem(double q) { return (q > 0.0 && q < 2147483647.0) ? (int32_t)q : 0; }
+
+int oracle_config_count(const mdpd_config_spec* s, uint32_t* n) {
+  const uint64_t all_unit = (uint64_t)s->head_unit + s->tail_unit;
+  const uint64_t tot = all_unit * s->amp_num + (s->with_solvent ? s->sol_num : 0u);
+  if (tot == 0 || tot > 0xffffffffu) return MDPD_ERR_ARG;
+  *n = (uint32_t)tot;
+  return MDPD_OK;
+}
+
+int oracle_make_config(const mdpd_config_spec* s, uint32_t n, double* pos, double* vel,
+                       uint32_t* id, uint32_t* prop, uint32_t* amp_id, uint32_t* unit,
+                       double* kin_temp) {
+  uint32_t nn = 0;
+  if (oracle_config_count(s, &nn) != MDPD_OK || nn != n) return MDPD_ERR_ARG;
+  o_cfg c = {s, s->head_unit + s->tail_unit, 0, 0, n, pos, id, prop, amp_id, unit};
+  const uint32_t amp_ptcl = c.all_unit * s->amp_num;
+  c.end_idx = amp_ptcl;
+  const double* L = s->box;
+  if (s->with_solvent && (s->mode == MDPD_CFG_RANDOM || s->mode == MDPD_CFG_ASYM_FLAT))
+    return MDPD_ERR_CONFIG;
+  int32_t sol_dim = 0;
+  double sol_rad = 0.0;
+  if (s->mode == MDPD_CFG_FLAT || s->mode == MDPD_CFG_ASYM_FLAT) {
+    /* MakeFlatSheet, axis = 1 */
+    const int axis = 1;
+    const uint32_t rat = s->mode == MDPD_CFG_ASYM_FLAT ? (uint32_t)(100 * s->in_out_rat) : 0xffffffffu;
+    const double eps = 1.0e-5;
+    double nv[3] = {0.0, 0.0, 0.0};
+    int flap = 1;
+    uint32_t a = 0;
+    while (c.idx < c.end_idx) {
+      o_saru r = o_saru3(a, CFG_TAG_AMP, s->seed);
+      double base[3];
+      for (int k = 0; k < 3; ++k) base[k] = L[k] * o_cfg_uni(&r, 1);
+      const double sign = flap ? 1.0 : -1.0;
+      base[axis] = 0.5 * L[axis] + ((c.all_unit - 1) * s->bond_leng + eps) * sign;
+      nv[axis] = -sign;
+      o_cfg_amph(&c, base, nv);
+      a++;
+      if (s->mode == MDPD_CFG_ASYM_FLAT) {
+        const uint32_t next = c.idx / c.all_unit;
+        if (((next % 100) == rat) || ((next % 100) == 0 && (next != 0))) flap ^= 1;
+      } else {
+        flap ^= 1;
+      }
+    }
+  } else if (s->mode == MDPD_CFG_RANDOM) {
+    uint32_t a = 0;
+    while (c.idx < c.end_idx) {
+      o_saru r = o_saru3(a, CFG_TAG_AMP, s->seed);
+      double base[3], nv[3];
+      for (int k = 0; k < 3; ++k) base[k] = L[k] * o_cfg_uni(&r, 0);
+      for (int k = 0; k < 3; ++k) nv[k] = o_cfg_uni(&r, 0);
+      const double norm_nv = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      for (int k = 0; k < 3; ++k) nv[k] /= norm_nv;
+      o_cfg_amph(&c, base, nv);
+      a++;
+    }
+  } else if (s->mode == MDPD_CFG_SPHERE) {
+    const double min_box = fmin(fmin(L[0], L[1]), L[2]);
+    if (!(s->sph_rad >= 0.0 && s->sph_rad <= 0.5 * min_box)) return MDPD_ERR_CONFIG;
+    const double up_the = s->upper_the * M_PI / 180.0;
+    const double q_thick = 0.5 * c.all_unit * s->bond_leng;
+    const double out_rad = s->sph_rad + q_thick, in_rad = s->sph_rad - q_thick;
+    double d_the_out = s->lip_len / out_rad, d_the_in = s->lip_len / in_rad;
+    const int32_t out_elem = o_cfg_elem(up_the / d_the_out), in_elem = o_cfg_elem(up_the / d_the_in);
+    d_the_out = up_the / out_elem;
+    d_the_in = up_the / in_elem;
+    o_cfg_theta_pass(&c, out_elem, d_the_out, 0.0, out_rad, -1.0);
+    o_cfg_theta_pass(&c, in_elem, d_the_in, 0.0, in_rad, 1.0);
+    const uint32_t residue = c.end_idx - c.idx;
+    if (residue > 0) {
+      o_cfg_theta_pass(&c, in_elem, d_the_in, 0.5, in_rad, 1.0);
+      if (c.end_idx != c.idx) return MDPD_ERR_CONFIG;
+    }
+    sol_dim = 3;
+    sol_rad = s->sph_rad;
+  } else if (s->mode == MDPD_CFG_CYLIND) {
+    const int axis = 2;
+    const double q_thick = 0.5 * c.all_unit * s->bond_leng;
+    const double out_rad = s->cyl_r + q_thick, in_rad = s->cyl_r - q_thick;
+    const double d_the_in = s->lip_len / in_rad;
+    const int32_t in_elem = o_cfg_elem(M_PI / d_the_in);
+    if (!(s->cyl_l >= 0.0 && s->cyl_l <= L[axis])) return MDPD_ERR_CONFIG;
+    if (!(s->cyl_r >= 0.0 && (2.0 * s->cyl_r + c.all_unit * s->bond_leng) < L[axis])) return MDPD_ERR_CONFIG;
+    const int32_t z_elem = o_cfg_elem(L[axis] / s->lip_len);
+    o_cfg_axis_pass(&c, z_elem, out_rad, -1.0);
+    o_cfg_axis_pass(&c, z_elem, in_rad, 1.0);
+    const uint32_t residue = c.end_idx - c.idx;
+    if (residue > 0) {
+      o_cfg_axis_pass(&c, in_elem, out_rad, -1.0);
+      if (c.end_idx != c.idx) return MDPD_ERR_CONFIG;
+    }
+    sol_dim = 2;
+    sol_rad = s->cyl_r;
+  } else {
+    return MDPD_ERR_ARG;
+  }
+  /* solvent (:282-321) */
+  if (s->with_solvent) {
+    const double h_thick = 0.5 * c.all_unit * s->bond_leng;
+    for (uint32_t k = 0; k < s->sol_num; ++k) {
+      const uint32_t i = amp_ptcl + k;
+      o_saru r = o_saru3(k, CFG_TAG_SOL, s->seed);
+      double p[3];
+      if (s->mode == MDPD_CFG_FLAT) {
+        double up[3] = {L[0], L[1], L[2]};
+        const double dw[3] = {0.0, 0.0, 0.0};
+        up[1] *= 0.5;
+        for (int a = 0; a < 3; ++a) p[a] = (up[a] - dw[a]) * o_cfg_uni(&r, 1) + dw[a];
+      } else {
+        uint32_t t = 0;
+        for (;; ++t) {
+          if (t == CFG_SOL_MAX_ATTEMPTS) return MDPD_ERR_CONFIG;
+          double d[3];
+          for (int a = 0; a < 3; ++a) p[a] = L[a] * o_cfg_uni(&r, 1);
+          for (int a = 0; a < 3; ++a) d[a] = p[a] - 0.5 * L[a];
+          double dist = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (sol_dim == 2) dist -= d[2] * d[2];
+          dist = sqrt(dist);
+          if (dist < sol_rad - h_thick) break;
+        }
+      }
+      memcpy(pos + 3 * (size_t)i, p, sizeof(p));
+      id[i] = i;
+      prop[i] = MDPD_SOLVENT;
+      unit[i] = MDPD_UNIT_NONE;
+      amp_id[i] = MDPD_UNIT_NONE;
+    }
+  }
+  /* GenVeloc :60-72 */
+  const double sd = sqrt(s->temperature);
+  for (uint32_t i = 0; i < n; ++i) {
+    o_saru r = o_saru3(i, CFG_TAG_VEL, s->seed);
+    for (int a = 0; a < 3; ++a) {
+      const double u1 = o_cfg_uni(&r, 1);
+      const double u2 = o_cfg_uni(&r, 1);
+      vel[3 * (size_t)i + a] = 0.0 + sd * sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    }
+  }
+  oracle_remove_cm_drift(n, vel);
+  /* CheckParticleConfigIsValid :420-452 */
+  double kin = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const double* v = vel + 3 * (size_t)i;
+    kin += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    for (int a = 0; a < 3; ++a) {
+      const double x = pos[3 * (size_t)i + a];
+      if (!(x <= L[a] && x >= 0.0)) return MDPD_ERR_CONFIG;
+    }
+  }
+  kin /= 3.0 * n;
+  if (kin_temp) *kin_temp = kin;
+  if (fabs(kin - s->temperature) >= 1.0e-2) return MDPD_ERR_CONFIG;
+  return MDPD_OK;
+}

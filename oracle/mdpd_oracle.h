@@ -25,6 +25,12 @@
  *   chemistry       invariants (ids, counts, geometry); the reference's MT
  *                   stream is replaced by keyed Saru draws: "parity unpinned"
  *                   against the reference's MT sequence (PS::MT is absent).
+ *   make_config     closed-form geometry of each mode (bond vectors, leaflet
+ *                   heights and radii, ring counts floor(2 pi r sin / lip),
+ *                   per-100 leaflet split), Box-Muller over the pinned Saru
+ *                   uniforms, zero drift, and the reference's error cases;
+ *                   keyed draws in place of the wall-clock-seeded MT stream
+ *                   ("parity unpinned" against that stream).
  */
 #ifndef MDPD_ORACLE_H
 #define MDPD_ORACLE_H
@@ -122,6 +128,13 @@ void oracle_sys_observe(oracle_sys* s, uint32_t sol_num,
 /* nsteps of the main loop body; returns 0 or 1 if chemistry asked to stop */
 int oracle_sys_run(oracle_sys* s, uint32_t nsteps);
 int oracle_num_threads(void);
+
+/* ---- initial configurations (src/config_maker.cpp) ---- */
+int oracle_config_count(const mdpd_config_spec* s, uint32_t* n);
+/* pos/vel [n][3]; returns MDPD_OK or the status the engine gives */
+int oracle_make_config(const mdpd_config_spec* s, uint32_t n, double* pos,
+                       double* vel, uint32_t* id, uint32_t* prop,
+                       uint32_t* amp_id, uint32_t* unit, double* kin_temp);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@ import subprocess
 
 import numpy as np
 
-from implicit_dpd_amd.capi import MdpdChemParams, MdpdObservables, MdpdParams
+from implicit_dpd_amd.capi import MdpdChemParams, MdpdConfigSpec, MdpdObservables, MdpdParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -61,6 +61,9 @@ _PROTOS = {
     "oracle_sys_observe": (None, [VP, C.c_uint32, P(MdpdObservables)]),
     "oracle_sys_run": (C.c_int, [VP, C.c_uint32]),
     "oracle_num_threads": (C.c_int, []),
+    "oracle_config_count": (C.c_int, [P(MdpdConfigSpec), _U32P]),
+    "oracle_make_config": (C.c_int, [P(MdpdConfigSpec), C.c_uint32, _DP, _DP, _U32P, _U32P, _U32P,
+                                     _U32P, _DP]),
 }
 
 _lib = None
@@ -298,3 +301,19 @@ class System:
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def make_config(spec):
+    """oracle_make_config: (status, dict of pos, vel [n, 3], ids, prop,
+    amp_id, unit, kin_temp)."""
+    n = C.c_uint32()
+    rc = lib().oracle_config_count(C.byref(spec), C.byref(n))
+    if rc:
+        return rc, None
+    n = n.value
+    pos = np.full((n, 3), np.nan)
+    vel = np.zeros((n, 3))
+    u = [np.full(n, 0xDEADBEEF, np.uint32) for _ in range(4)]
+    kt = C.c_double()
+    rc = lib().oracle_make_config(C.byref(spec), n, _d(pos), _d(vel), *[_u(a) for a in u], C.byref(kt))
+    return rc, dict(pos=pos, vel=vel, ids=u[0], prop=u[1], amp_id=u[2], unit=u[3], kin_temp=kt.value)
