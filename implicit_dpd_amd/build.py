@@ -59,24 +59,29 @@ def build(verbose=True):
     return OUT
 
 
-DRIVER_SRC = os.path.join(HERE, "driver", "mdpd_main.cpp")
 DRIVER = os.path.join(HERE, "bin", "mdpd_main")
+CONFIG_MAKER = os.path.join(HERE, "bin", "config_maker")
+# host programs over the C++ facade: the run driver (src/main.cpp) and the
+# initial-configuration program (src/config_maker.cpp)
+PROGRAMS = {DRIVER: os.path.join(HERE, "driver", "mdpd_main.cpp"),
+            CONFIG_MAKER: os.path.join(HERE, "driver", "config_maker.cpp")}
 
 
 def build_driver(verbose=True):
-    """The stand-alone host driver (C++ facade over the C ABI), linked
+    """The stand-alone host programs (C++ facade over the C ABI), linked
     against the in-tree library with an rpath to it."""
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
     cxx = os.environ.get("CXX", "g++")
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
-           DRIVER_SRC, "-o", DRIVER + ".tmp", "-L", HERE, "-lmdpd",
-           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(DRIVER + ".tmp", DRIVER)
-    if verbose:
-        print(f"built {DRIVER}")
+    for out, src in PROGRAMS.items():
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+               src, "-o", out + ".tmp", "-L", HERE, "-lmdpd",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host program build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print(f"built {out}")
     return DRIVER
 
 

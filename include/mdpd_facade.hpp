@@ -12,6 +12,7 @@
 //   ForceBonded<Tpsys>::CalcListedForce(sys, vir)       (bdf_calculator.hpp:191)
 //   ChemManager<FPDPD>::RandomChemEvent(sys, topol, param) (chemmanager.hpp:244)
 //   Observer<Tpsys> (observer.hpp:7-178): the same files and columns
+//   ConfigMaker (src/config_maker.cpp:13-533)   mdpd::ConfigMaker
 //
 // Errors behave as in the reference: a failed call prints the engine's
 // message and aborts (PS::Abort).  Header-only; link with -lmdpd.
@@ -22,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -215,6 +217,70 @@ inline uint32_t ReadXYZForm(const std::string& fname, std::vector<FPDPD>& v) {
   std::fclose(fp);
   return t;
 }
+
+// ConfigMaker (src/config_maker.cpp:13-533): config_param.txt in the target
+// directory -> init_config.xyz, the beads generated on the device.  As in the
+// reference, Initialize() seeds from time(NULL) (:452-454); pass a seed to
+// reproduce a configuration.  GenParticles(system) generates straight into
+// an engine context instead of host records.
+class ConfigMaker {
+ public:
+  explicit ConfigMaker(std::string cdir) : cdir_(std::move(cdir)) {}
+  void Initialize(uint32_t seed = (uint32_t)std::time(nullptr)) { seed_ = seed; }
+  void LoadParam() {
+    Check(mdpd_read_config_param(cdir_.c_str(), &spec_), nullptr, "mdpd_read_config_param");
+    spec_.seed = seed_;
+  }
+  void GenParticles() {
+    uint32_t n = 0;
+    Check(mdpd_config_count(&spec_, &n), nullptr, "mdpd_config_count");
+    std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n);
+    std::vector<uint32_t> id(n), prop(n), amp(n), unit(n);
+    Check(mdpd_make_config_host(&spec_, n, pos.data(), vel.data(), id.data(), prop.data(),
+                                amp.data(), unit.data(), &kin_temp_),
+          nullptr, "mdpd_make_config_host");
+    prtcls_.assign(n, FPDPD{});
+    for (uint32_t i = 0; i < n; ++i) {
+      FPDPD& q = prtcls_[i];
+      q.id = id[i];
+      q.prop = prop[i];
+      q.amp_id = amp[i];
+      q.unit = unit[i];
+      for (int a = 0; a < 3; ++a) {
+        q.pos[a] = pos[3 * (size_t)i + a];
+        q.vel[a] = q.vel_buf[a] = vel[3 * (size_t)i + a];
+      }
+    }
+  }
+  void GenParticles(ParticleSystem& system) {
+    Check(mdpd_make_config(system.ctx(), &spec_, &kin_temp_), system.ctx(), "mdpd_make_config");
+  }
+  void DumpParticleConfig() const {
+    const std::string f = cdir_ + "/init_config.xyz";
+    FILE* fp = std::fopen(f.c_str(), "w");
+    if (!fp) Abort("cannot open " + f);
+    WriteXYZForm(prtcls_, 0, fp);
+    std::fclose(fp);
+  }
+  void ShowSystemParameter() const {
+    static const char* modes[] = {"flat", "asym_flat", "sphere", "cylind", "random"};
+    std::printf("Show ConfigMaker Paramter\nTempera %g\nbox_leng %g %g %g\namp_num %u\n"
+                "head_unit %u\ntail_unit %u\nall_unit %u\nbond_leng %g\nmode %s\n",
+                spec_.temperature, spec_.box[0], spec_.box[1], spec_.box[2], spec_.amp_num,
+                spec_.head_unit, spec_.tail_unit, spec_.head_unit + spec_.tail_unit,
+                spec_.bond_leng, modes[spec_.mode]);
+  }
+  const std::vector<FPDPD>& particles() const { return prtcls_; }
+  const mdpd_config_spec& spec() const { return spec_; }
+  double kinetic_temperature() const { return kin_temp_; }
+
+ private:
+  std::string cdir_;
+  uint32_t seed_ = 0;
+  mdpd_config_spec spec_{};
+  std::vector<FPDPD> prtcls_;
+  double kin_temp_ = 0.0;
+};
 
 // Observer (observer.hpp:7-178): same file names, columns and formats.  One
 // device reduction (mdpd_observe) serves every quantity of a step.

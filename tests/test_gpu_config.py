@@ -131,3 +131,46 @@ def test_membrane_bilayer_full_size(oracle):
     assert rc == 0
     g = cm.make_config(s)
     _compare(g, r, "flat", (100.0, 100.0, 40.0))
+
+
+def test_config_maker_programs_and_run_driver(engine_mod, tmp_path):
+    """The config_maker programs (C++ over the facade, and the Python CLI) on
+    the reference's own config_param.txt write the same init_config.xyz, the
+    beads of mdpd_make_config_host; the run driver then starts from it."""
+    import os
+    import shutil
+    import subprocess
+
+    from implicit_dpd_amd import build, configs
+    from implicit_dpd_amd.fpdpd import read_xyz
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config_param_rho1.0.txt")
+    d1, d2 = tmp_path / "cxx", tmp_path / "py"
+    for d in (d1, d2):
+        d.mkdir()
+        shutil.copy(gold, d / "config_param.txt")
+    # seed 8: a draw whose kinetic temperature passes the 1e-2 check at 4096 beads
+    print(f"[config_maker] {build.CONFIG_MAKER} {d1} 8", flush=True)
+    r = subprocess.run([build.CONFIG_MAKER, str(d1), "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "mode random" in r.stdout
+    assert cm.main([str(d2), "--seed", "8"]) == 0
+    t1 = (d1 / "init_config.xyz").read_text()
+    assert t1 == (d2 / "init_config.xyz").read_text()
+    rec, t = read_xyz(str(d1 / "init_config.xyz"))
+    assert t == 0 and len(rec) == 4096
+    h = cm.make_config(cm.read_config_param(str(d1), seed=8))
+    assert np.abs(rec["pos"] - h["pos"]).max() < 1e-13
+    assert np.abs(rec["vel"] - h["vel"]).max() < 1e-13 and np.array_equal(rec["vel"], rec["vel_buf"])
+    for k, kk in (("id", "ids"), ("prop", "prop"), ("amp_id", "amp_id"), ("unit", "unit")):
+        assert np.array_equal(rec[k], h[kk])
+    # the run driver reads it with a run_param.txt of the same box
+    configs.write_run_param(str(d1), (16.0, 16.0, 16.0), 1024, dt=0.005, all_time=6, step_mic=3,
+                            step_mac=3, core=(0,))
+    print(f"[driver] {build.DRIVER} {d1} --seed 3", flush=True)
+    r = subprocess.run([build.DRIVER, str(d1), "--seed", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    kin = np.loadtxt(str(d1 / "kin_temp.txt"))
+    assert kin.shape == (2, 4) and np.isfinite(kin).all()  # observations at t = 0, 3
+    fin, tf = read_xyz(str(d1 / "fin_config.xyz"))
+    assert tf == 6 and len(fin) == 4096
