@@ -270,6 +270,9 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   d.W = c->tune.nlist_width;
   d.n_rep = n_rep;
   d.n_dens = c->nd;
+  d.rd_le_1 = 1;
+  for (int r = 0; r < n_rep; ++r)
+    if (!(p[r].rd <= 1.0)) d.rd_le_1 = 0;
   d.n_amp = 0;
   if (hipMemcpyAsync(d.reps, c->R.data(), sizeof(Replica) * n_rep, hipMemcpyHostToDevice, c->st) !=
           hipSuccess ||

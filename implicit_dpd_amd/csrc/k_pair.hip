@@ -76,7 +76,9 @@ __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s)
 // list; vcut2 carries a margin over (1 + skin)^2 that absorbs the FP32
 // error, and cells are at least 1 + skin wide, so no pair that can come
 // inside the cutoff before the next rebuild is left out.
-template <int ND, bool MULTI, bool BUILD>
+// RDLE1: every rd <= 1, so a pair strictly inside cut_hi and outside the
+// guard bands is inside the cutoff 1 (no compare needed).
+template <int ND, bool MULTI, bool BUILD, bool RDLE1>
 __global__ void __launch_bounds__(256) k_density(DevState d) {
   if (BUILD && *d.rebuild == 0u) return;
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -123,7 +125,10 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t* S = d.cell_start + rowq;
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      const uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
+      // per-lane registers: a pair-list entry then selects its code without
+      // first copying the three run constants into vector registers
+      uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
+      asm volatile("" : "+v"(C0), "+v"(C1), "+v"(C2));
       for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
         // run [ta, tb); slots >= b2 are in the right cell, >= b1 the own cell
         uint32_t ta, tb, b1, b2;
@@ -142,7 +147,7 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           const bool g2 = t >= b2, g1 = t >= b1;
           const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
-          const float r2 = dx * dx + dy * dy + dz * dz;
+          float r2 = dx * dx + dy * dy + dz * dz;
           if (BUILD) {
             if (r2 >= cut_hi || t == s) continue;
             const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
@@ -157,10 +162,16 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
             continue;
           }
           if (r2 >= cut_hi) continue;
-          bool in_rc = r2 < 1.0f, in_rd = r2 < rd2f;
+          // Outside the guard bands the FP32 r^2 decides, and there the
+          // clamp of w at 0 is the rd test (r^2 > rd^2 + kGuard gives
+          // 1 - r/rd < 0 far beyond the FP32 error), so the common path
+          // has no rd compare; with RDLE1 it has no cutoff compare either.
+          bool in_rc = RDLE1 ? true : (r2 < 1.0f);
           // next to a cutoff, or (near-)coincident: decide in FP64; the
           // bead itself (t == s, r2 == 0) lands here and is dropped
-          if (fabsf(r2 - 1.0f) <= kGuard || fabsf(r2 - rd2f) <= kGuard || r2 < 1e-6f) {
+          const bool near = (fabsf(r2 - 1.0f) <= kGuard) | (fabsf(r2 - rd2f) <= kGuard) |
+                            (r2 < 1e-6f);
+          if (near) {
             if (t == s) continue;
             // opaque copy: keeps the compiler from stepping an FP64 address
             // along with t in every iteration for this rare path
@@ -168,9 +179,10 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
             asm volatile("" : "+v"(tt));
             const double r2d = fp64_r2(d, R, s, tt);
             in_rc = r2d < 1.0;
-            in_rd = r2d < R.rd2 && r2d != 0.0;
+            // outside rd (or coincident): a value that makes w = 0
+            if (!(r2d < R.rd2 && r2d != 0.0)) r2 = 2.0f * rd2f + 1.0f;
           }
-          if (in_rd) {
+          {
             const float rr = __builtin_amdgcn_sqrtf(r2);
             const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
             const float v = crho * w * w;
@@ -189,8 +201,10 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
             e0 = e1;
             e1 = e2;
             e2 = e;
-            if ((cnt & 3u) == 3u && cnt < W)
-              lst[(size_t)(cnt >> 2) * 64u] = make_uint4(out, e0, e1, e2);
+            // past W entries the last chunk is overwritten; the overflow
+            // flag below fails the call
+            if ((cnt & 3u) == 3u)
+              lst[(size_t)min(cnt >> 2, (W >> 2) - 1u) * 64u] = make_uint4(out, e0, e1, e2);
             ++cnt;
           }
         }
@@ -220,17 +234,21 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
 
 template <int ND, bool MULTI>
 static void dens_launch(const DevState& d, int block, hipStream_t st) {
-  hipLaunchKernelGGL((k_density<ND, MULTI, false>), dim3(div_up(d.n, block)), dim3(block), 0, st,
-                     d);
+  if (d.rd_le_1)
+    hipLaunchKernelGGL((k_density<ND, MULTI, false, true>), dim3(div_up(d.n, block)), dim3(block), 0,
+                       st, d);
+  else
+    hipLaunchKernelGGL((k_density<ND, MULTI, false, false>), dim3(div_up(d.n, block)), dim3(block),
+                       0, st, d);
 }
 
 void launch_skin_build(const DevState& d, hipStream_t st) {
   if (!d.n) return;
   const int block = 128;
   if (d.n_rep > 1)
-    hipLaunchKernelGGL((k_density<1, true, true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_density<1, true, true, false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
   else
-    hipLaunchKernelGGL((k_density<1, false, true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_density<1, false, true, false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
 }
 
 // Density pass of skin mode: the same decisions and sums as k_density, over

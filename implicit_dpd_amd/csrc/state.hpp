@@ -68,6 +68,7 @@ struct DevState : Master {
   uint32_t* rep_cell_off; // [n_rep + 1]
   int32_t n_rep;
   int32_t n_dens;
+  int32_t rd_le_1;      // every replica's rd <= 1 (the density pass's cutoff test)
   // bonded
   uint32_t* topo;       // [4][amp_cap] chain bead slots, amp-major
   double4* chainpos;    // [4 amp_cap] FP64 positions of chain beads, chain-major
