@@ -386,8 +386,18 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   // single system: the coefficients come from a block-shared table indexed
   // by the type pair instead of per-lane selects
   __shared__ float4 ptab[18];
-  if (!MULTI && NS > 1) {
-    fill_pair_table(d.reps[0], ptab);
+  // single system: the image offset o * h of each stencil code (o in
+  // {-1, 0, 1} per axis, exact in FP32), so pi - o h rounds once as before
+  __shared__ float4 otab[64];
+  if (!MULTI) {
+    if (NS > 1) fill_pair_table(d.reps[0], ptab);
+    if (threadIdx.x < 64u) {
+      const uint32_t k = threadIdx.x;
+      const Replica& R0 = d.reps[0];
+      otab[k] = make_float4((float)((int)(k & 3u) - 1) * R0.hf[0],
+                            (float)((int)((k >> 2) & 3u) - 1) * R0.hf[1],
+                            (float)((int)(k >> 4) - 1) * R0.hf[2], 0.0f);
+    }
     __syncthreads();
   }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -426,9 +436,17 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     const uint32_t code = e >> kCodeShift;
     const float4 pj = d.sb[t].pos;
     const float4 vj = d.sb[t].vel;
-    float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
-    float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
-    float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    float dx, dy, dz;
+    if (!MULTI) {
+      const float4 o = otab[code];
+      dx = (pi.x - o.x) - pj.x;
+      dy = (pi.y - o.y) - pj.y;
+      dz = (pi.z - o.z) - pj.z;
+    } else {
+      dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+      dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
+      dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    }
     float r2 = dx * dx + dy * dy + dz * dz;
     if (r2 < kClose2) {
       // a near-coincident pair: the FP32 coordinates' absolute error (a few
@@ -492,7 +510,9 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
 #pragma unroll
       for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
     }
-    const float wm = rdm1 ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
+    // with rdm = 1, 1 - r * 1 is 1 - r bit for bit, so only the reference
+    // envelope (whose wc may come from FP64 near the cutoff) needs wc itself
+    const float wm = (ENV == 0 && rdm1) ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
     const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
     const float fs = (cons_c * wc + mbs * wm + sg * wR * xi - gm * wD * rv * rinv) * rinv;
     fx += fs * dx;
