@@ -111,8 +111,8 @@ __device__ __forceinline__ void bonded_kick_bead(const DevState& d, const Replic
   if (kick) {
     const double dt = R.dt;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      d.vel[c * cap + s] = d.vbuf[c * cap + s] + dt * 0.5 * (double)af[c];
+    for (int c = 0; c < 3; ++c)  // vel_buf's last use in the step: streaming load
+      d.vel[c * cap + s] = __builtin_nontemporal_load(&d.vbuf[c * cap + s]) + dt * 0.5 * (double)af[c];
   }
 }
 

@@ -350,10 +350,13 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   double xs[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    double x = d.pos[a * cap + i];
-    double vb, dl = d.delta[a * cap + i];
+    // the old set's position, delta_sumr, velocity and acceleration are
+    // read for the last time in this step: streaming loads
+    double x = __builtin_nontemporal_load(&d.pos[a * cap + i]);
+    double vb, dl = __builtin_nontemporal_load(&d.delta[a * cap + i]);
     if (drift) {
-      const DriftAxis o = drift_axis(R, a, x, d.vel[a * cap + i], (double)d.acc[a * cap + i], dl);
+      const DriftAxis o = drift_axis(R, a, x, __builtin_nontemporal_load(&d.vel[a * cap + i]),
+                                     (double)__builtin_nontemporal_load(&d.acc[a * cap + i]), dl);
       x = o.p;
       dl = o.dl;
       vb = o.vb;
@@ -361,9 +364,12 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
       vb = d.vbuf[a * cap + i];
     }
     xs[a] = x;
-    d.alt.pos[a * cap + s] = x;
+    // positions and delta_sumr are next read by the following step's
+    // drift and permutation: streaming stores keep them from displacing
+    // this step's pair-pass data in L2 (vel_buf is read by this step's kick)
+    __builtin_nontemporal_store(x, &d.alt.pos[a * cap + s]);
     d.alt.vbuf[a * cap + s] = vb;
-    d.alt.delta[a * cap + s] = dl;
+    __builtin_nontemporal_store(dl, &d.alt.delta[a * cap + s]);
     // skin mode: the displacement test of the coming steps starts here
     if (skin) d.dref[a * cap + s] = dl;
     l[a] = (float)(x - (double)cc[a] * R.h[a]);
@@ -374,14 +380,14 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     }
     v[a] = (float)(predicted ? vb : d.vel[a * cap + i]);
   }
-  const uint32_t id = d.id[i], pr = d.prop[i];
-  d.alt.id[s] = id;
+  const uint32_t id = __builtin_nontemporal_load(&d.id[i]), pr = d.prop[i];
+  __builtin_nontemporal_store(id, &d.alt.id[s]);
   d.alt.prop[s] = pr;
   d.alt.amp_id[s] = d.amp_id[i];
   d.alt.unit[s] = d.unit[i];
-  d.alt.orig[s] = d.orig[i];
-  const uint32_t ch = d.chain[i];
-  d.alt.chain[s] = ch;
+  __builtin_nontemporal_store(__builtin_nontemporal_load(&d.orig[i]), &d.alt.orig[s]);
+  const uint32_t ch = __builtin_nontemporal_load(&d.chain[i]);
+  __builtin_nontemporal_store(ch, &d.alt.chain[s]);
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(xs[0], xs[1], xs[2], 0.0);
   const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
   d.spos[s] = ps;
