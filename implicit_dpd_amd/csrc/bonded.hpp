@@ -88,9 +88,11 @@ __device__ __forceinline__ void chain_terms(const DevState& d, const Replica& R,
 // Bead s: adds the bonded force of its own chain to its FP32 acceleration
 // af (the value the force pass stored), stores it, and kicks.  The chain
 // virial is counted once, by the chain's first bead, into vir_bead[3s..].
+// vout (may be null): the kicked velocity.
 __device__ __forceinline__ void bonded_kick_bead(const DevState& d, const Replica& R, uint32_t s,
                                                  float af[3], bool virial, bool kick,
-                                                 double* __restrict__ vir_bead) {
+                                                 double* __restrict__ vir_bead,
+                                                 double* vout = nullptr) {
 #pragma clang fp contract(off)
   const uint32_t cap = d.cap;
   const uint32_t ch = d.chain[s];
@@ -111,8 +113,11 @@ __device__ __forceinline__ void bonded_kick_bead(const DevState& d, const Replic
   if (kick) {
     const double dt = R.dt;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)  // vel_buf's last use in the step: streaming load
-      d.vel[c * cap + s] = __builtin_nontemporal_load(&d.vbuf[c * cap + s]) + dt * 0.5 * (double)af[c];
+    for (int c = 0; c < 3; ++c) {  // vel_buf's last use in the step: streaming load
+      const double v = __builtin_nontemporal_load(&d.vbuf[c * cap + s]) + dt * 0.5 * (double)af[c];
+      d.vel[c * cap + s] = v;
+      if (vout) vout[c] = v;
+    }
   }
 }
 

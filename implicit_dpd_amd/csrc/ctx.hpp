@@ -40,9 +40,9 @@ struct mdpd_ctx {
   // skin mode: the skin list, its cells and the displacement reference match
   // the bead storage (cleared by every call that moves or reorders beads)
   bool skin_valid = false;
-  // captured steps by (storage parity, virial, cm-drift step)
-  hipGraphExec_t gexec[2][4] = {{nullptr, nullptr, nullptr, nullptr},
-                                {nullptr, nullptr, nullptr, nullptr}};
+  // captured steps by storage parity and variant (1: virial step, 2: CM-drift
+  // step, 4: the step computes its own cell keys)
+  hipGraphExec_t gexec[2][8] = {};
   int parity = 0;            // which buffer set is current (flips every rebuild)
   uint32_t* inv_orig = nullptr;  // scratch [cap]
   bool profiling = false;

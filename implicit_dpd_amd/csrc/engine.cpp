@@ -288,8 +288,11 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   return MDPD_OK;
 }
 
-// one step of main.cpp:167-196 on the context stream
-void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
+// one step of main.cpp:167-196 on the context stream.  first: the cell keys
+// of the drifted positions come from a key pass (the first step of a run,
+// and the step after a CM-drift removal, which changed the velocities);
+// otherwise the previous step's force epilogue wrote them.
+void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first) {
   const DevState& d = c->d;
   hipStream_t st = c->st;
   const bool prof = c->profiling;
@@ -338,8 +341,9 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
     mark(10);
     return;
   }
-  // the cell histogram is zero here: allocated zeroed and cleared by every scan
-  launch_step_keys(d, st);
+  // the cell histogram is zero before the force pass that fills it (or this
+  // key pass): allocated zeroed and cleared by every scan
+  if (first) launch_step_keys(d, st);
   mark(1);
   launch_scan_cells(d, st);
   launch_scatter(d, st);
@@ -354,7 +358,10 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm) {
   // the force pass ends with ForceBonded::CalcListedForce and the kick of
   // each bead (bead-centric: every bead evaluates its own chain)
   // ... and, with any reactive replica, the head conversion of each bead
-  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true, c->any_conv);
+  // ... and the next step's cell keys, unless this is a run's last step (the
+  // host may change the state before the next run) or a CM step
+  launch_force(d, c->ns, c->nd, c->env, virial, c->tune.block_force, st, true, c->any_conv,
+               !virial && !cm);
   mark(5);
   if (virial) launch_bead_vir_reduce(d, c->vir_bonded, st);
   mark(6);
@@ -393,11 +400,13 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   // the device clock holds the previous step's time; each step's first
   // kernel advances it
   if (nsteps) launch_fill_u32(c->d.d_time, 1, c->time - 1u, c->st);
+  bool first = true;
   for (uint32_t k = 0; k < nsteps; ++k) {
+    // the last step of a run is its virial step
     const bool virial = (k + 1 == nsteps);
     const bool cm = (c->time % c->cm_corr) == 0;
     if (c->tune.use_graph && !c->profiling) {
-      const int v = (virial ? 1 : 0) | (cm ? 2 : 0);
+      const int v = (virial ? 1 : 0) | (cm ? 2 : 0) | (first && !skin ? 4 : 0);
       // (a skin-mode step keeps the storage where it is: no parity flip)
       const int p = c->parity;
       if (!c->gexec[p][v]) {
@@ -405,7 +414,7 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
         // running it does
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, virial, cm);
+        enqueue_step(c, virial, cm, first);
         HIPCHK(c, hipStreamEndCapture(c->st, &g));
         HIPCHK(c, hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0));
         hipGraphDestroy(g);
@@ -414,8 +423,9 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
       }
       HIPCHK(c, hipGraphLaunch(c->gexec[p][v], c->st));
     } else {
-      enqueue_step(c, virial, cm);
+      enqueue_step(c, virial, cm, first);
     }
+    first = cm;
     c->time++;
   }
   c->drifted = false;

@@ -25,6 +25,7 @@
 #include "saru.hpp"
 #include "state.hpp"
 #include "bonded.hpp"
+#include "step_dev.hpp"
 
 namespace mdpd {
 
@@ -380,9 +381,13 @@ __device__ __forceinline__ void fill_pair_table(const Replica& R, float4* tab) {
   }
 }
 
+// keys (step mode only): the epilogue also bins the bead's position after
+// the next step's drift (the same FP64 drift as k_drift, from the kicked
+// velocity and the acceleration just stored) into d.key and the cell
+// histogram, which the next step's scan reads instead of a key pass.
 template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
 __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead,
-                                               int conv) {
+                                               int conv, int keys) {
   // single system: the coefficients come from a block-shared table indexed
   // by the type pair instead of per-lane selects
   __shared__ float4 ptab[18];
@@ -401,7 +406,12 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     __syncthreads();
   }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  const bool active = s < d.n;
+  // with keys every lane of the wave reaches the aggregated histogram
+  // increment at the end
+  if (!active && !keys) return;
+  uint32_t key = 0;
+  if (active) {
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const uint32_t seed3 = R.pair_seed + *d.d_time;
@@ -541,7 +551,22 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     // ForceBonded::CalcListedForce and kick for this bead, from the FP32
     // acceleration just summed (the same value the stand-alone pass reads)
     float af[3] = {fx, fy, fz};
-    bonded_kick_bead(d, R, s, af, VIRIAL, true, vir_bead);
+    double vk[3];
+    bonded_kick_bead(d, R, s, af, VIRIAL, true, vir_bead, vk);
+    if (keys) {
+      // the next step's drifted position (drift_and_predict,
+      // driftkick.hpp:5-30) and its cell, as k_drift computes them
+      double pn[3];
+      bool fin = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        pn[a] = drift_axis(R, a, d.pos[a * d.cap + s], vk[a], (double)af[a], 0.0).p;
+        fin = fin && isfinite(pn[a]) && isfinite(vk[a]);
+      }
+      if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
+      key = cell_of(R, pn);
+      d.key[s] = key;
+    }
     if (conv && d.unit[s] == 0u && d.prop[s] == (uint32_t)R.conv_from && R.conv_p > 0.0) {
       // the step's head conversion for this bead (same draw as
       // k_head_conv); only the master type changes, and the pair passes
@@ -554,37 +579,40 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     d.acc[d.cap + s] = fy;
     d.acc[2 * d.cap + s] = fz;
   }
+  }  // active
+  if (keys) (void)agg_inc(d.cell_cnt, key, active);
 }
 
 template <int NS, int ND, int ENV>
 static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb,
-                          int cv) {
+                          int cv, int ky) {
   const dim3 g(div_up(d.n, block)), b(block);
   const bool multi = d.n_rep > 1;
   if (multi) {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb, cv);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb, cv, ky);
   } else {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb, cv);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb, cv);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb, cv, ky);
   }
 }
 
 template <int NS, int ND>
 static void force_launch2(const DevState& d, int env, bool virial, int block, hipStream_t st,
-                          double* vb, int cv) {
-  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st, vb, cv);
-  else force_launch3<NS, ND, 1>(d, virial, block, st, vb, cv);
+                          double* vb, int cv, int ky) {
+  if (env == 0) force_launch3<NS, ND, 0>(d, virial, block, st, vb, cv, ky);
+  else force_launch3<NS, ND, 1>(d, virial, block, st, vb, cv, ky);
 }
 
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st, bool fuse, bool conv) {
+                  hipStream_t st, bool fuse, bool conv, bool keys) {
   if (!d.n) return;
   double* vb = fuse ? bead_vir_buffer(d) : nullptr;
   const int cv = (fuse && conv) ? 1 : 0;
-  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st, vb, cv);
-  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st, vb, cv);
-  else force_launch2<3, 1>(d, env, virial, block, st, vb, cv);
+  const int ky = (fuse && keys) ? 1 : 0;
+  if (nd == 3) force_launch2<3, 3>(d, env, virial, block, st, vb, cv, ky);
+  else if (ns == 1) force_launch2<1, 1>(d, env, virial, block, st, vb, cv, ky);
+  else force_launch2<3, 1>(d, env, virial, block, st, vb, cv, ky);
 }
 
 // ---------------------------------------------------------------- pair listing

@@ -10,6 +10,7 @@
 #include "saru.hpp"
 #include "state.hpp"
 #include "bonded.hpp"
+#include "step_dev.hpp"
 
 namespace mdpd {
 
@@ -39,61 +40,11 @@ __global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
 }
 
 // ---------------------------------------------------------------- cells
-__device__ __forceinline__ uint32_t cell_of(const Replica& R, const double p[3]) {
-  int c[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int x = (int)floor(p[a] * R.inv_h[a]);
-    x = x < 0 ? 0 : x;
-    x = x >= R.nc[a] ? R.nc[a] - 1 : x;
-    c[a] = x;
-  }
-  return R.cell_off + ((uint32_t)c[2] * (uint32_t)R.nc[1] + (uint32_t)c[1]) * (uint32_t)R.nc[0] +
-         (uint32_t)c[0];
-}
-
-// Wave-aggregated "slot = base[key]++": lanes of a contiguous run holding
-// the same key (bead storage is in cell order, so runs are the rule) share a
-// single atomic issued by the run's first lane; members get consecutive
-// slots in lane order.  Inactive lanes break runs and take no slot.
-__device__ __forceinline__ uint32_t agg_inc(uint32_t* base, uint32_t key, bool active) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t kprev = __shfl_up(key, 1, 64);
-  const int aprev = __shfl_up(active ? 1 : 0, 1, 64);
-  const bool leader = active && (lane == 0 || !aprev || kprev != key);
-  const unsigned long long stop = __ballot(leader || !active);
-  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int start = 63 - __clzll(stop & upto);  // my run's leader (or me)
-  const unsigned long long above = stop & ~upto;
-  const int end = above ? (__ffsll((long long)above) - 1) : 64;
-  uint32_t old = 0;
-  if (leader) old = atomicAdd(&base[key], (uint32_t)(end - lane));
-  const uint32_t b = __shfl(old, start, 64);
-  return b + (uint32_t)(lane - start);
-}
 
 // drift_and_predict, driftkick.hpp:5-30.  vel is left as the last kicked
 // velocity; the predicted velocity (vel_buf == vel of the reference after
 // the drift) goes to vbuf.  With `keys` the cell key and the cell
 // histogram of the rebuild are produced in the same pass.
-// One axis of drift_and_predict (driftkick.hpp:5-30) in FP64 without
-// contraction: the wrapped position, delta_sumr and the predicted velocity.
-// Every kernel that drifts uses this, so they agree bit for bit.
-struct DriftAxis {
-  double p, dl, vb;
-};
-__device__ __forceinline__ DriftAxis drift_axis(const Replica& R, int a, double x, double v,
-                                                double ac, double dl) {
-#pragma clang fp contract(off)
-  const double dt = R.dt;
-  const double dr = dt * (v + 0.5 * dt * ac);
-  const double tr = x + dr;
-  DriftAxis o;
-  o.dl = dl + dr;
-  o.p = tr - floor(tr * R.inv_box[a]) * R.box[a];
-  o.vb = v + 0.5 * dt * ac;
-  return o;
-}
 
 // store = 0 (the step): only the cell key and histogram of the drifted
 // position; the permutation that follows drifts each bead again as it moves
@@ -147,7 +98,7 @@ void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t s
 }
 void launch_step_keys(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0);
+  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 0);
 }
 
 __global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
@@ -332,8 +283,11 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
 // drift: the master set holds the state before this step's drift; the bead
 // is drifted here (same arithmetic as k_drift) and written once, drifted,
 // to its new slot.
+// tick: the step's clock advances here (no kernel of the step before the
+// force pass reads it)
 __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin,
-                                                 int drift) {
+                                                 int drift, int tick) {
+  if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
   if (skin && *d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
@@ -398,11 +352,11 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
   hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
-                     full ? 1 : 0, 0, 0);
+                     full ? 1 : 0, 0, 0, 0);
 }
 void launch_step_permute(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
 }
 
 // chain bead slots follow the beads to their new slots
@@ -771,7 +725,7 @@ void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
 }
 void launch_permute_gated(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0, 0);
 }
 
 // The rebuild's permutation wrote the other buffer set; copy it back so the

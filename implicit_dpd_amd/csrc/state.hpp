@@ -89,10 +89,10 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st,
                   bool tick = false);
 void launch_keys(const DevState& d, hipStream_t st);
-// the step's first kernel: clock, and cell key + histogram of each bead's
-// drifted position (the state itself is drifted by launch_step_permute)
+// cell key + histogram of each bead's drifted position (the state itself
+// is drifted by launch_step_permute)
 void launch_step_keys(const DevState& d, hipStream_t st);
-// the step's permutation: drift + move into the other set in cell order
+// the step's permutation: clock, drift + move into the other set in cell order
 void launch_step_permute(const DevState& d, hipStream_t st);
 void launch_scan_cells(const DevState& d, hipStream_t st);
 void launch_scatter(const DevState& d, hipStream_t st);
@@ -107,8 +107,10 @@ void launch_density(const DevState& d, int ns, int nd, int block, hipStream_t st
 // fuse: the force pass ends with the bead's bonded terms and the kick
 // (virial steps: per-bead chain virials for launch_bead_vir_reduce)
 // conv (with fuse): the head conversion of the bead ends the epilogue
+// keys (with fuse): the epilogue also writes the cell key and histogram of
+// the bead's position after the next step's drift (launch_step_keys' work)
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
-                  hipStream_t st, bool fuse = false, bool conv = false);
+                  hipStream_t st, bool fuse = false, bool conv = false, bool keys = false);
 double* bead_vir_buffer(const DevState& d);
 void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st);
 void launch_kick(const DevState& d, hipStream_t st);

@@ -198,3 +198,25 @@ def test_run_is_deterministic_and_graph_matches_eager(engine_mod):
     for k in ("pos", "vel", "acc"):
         assert np.array_equal(res[0][k], res[1][k])
         assert np.array_equal(res[0][k], res[2][k])
+
+
+def test_split_runs_and_cm_steps_are_bit_identical(engine_mod):
+    """Cell keys of a step come from the previous step's force pass, except
+    on a run's first step and after a CM-drift removal, which bin the beads
+    in a key pass of their own: one 12-step run, the same 12 steps as runs of
+    5 + 4 + 3, and the kernel-by-kernel launches all give the same
+    trajectory bit for bit, with a CM-drift removal every third step."""
+    name, p0, s = SYSTEMS[2]
+    p = type(p0).from_buffer_copy(p0)
+    p.cm_corr = 3
+    res = []
+    for graph, split in ((1, (12,)), (1, (5, 4, 3)), (0, (12,)), (0, (1, 2, 9))):
+        e = _engine(engine_mod, p, s, time=4)
+        e.set_tuning(use_graph=graph)
+        for k in split:
+            e.run(k)
+        res.append(e.download())
+        e.close()
+    for r in res[1:]:
+        for k in ("pos", "vel", "vel_buf", "acc", "delta"):
+            assert np.array_equal(res[0][k], r[k]), k
