@@ -114,7 +114,8 @@ __device__ __forceinline__ void bonded_kick_bead(const DevState& d, const Replic
     const double dt = R.dt;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {  // vel_buf's last use in the step: streaming load
-      const double v = __builtin_nontemporal_load(&d.vbuf[c * cap + s]) + dt * 0.5 * (double)af[c];
+      const double vb = d.n >= kStreamMin ? ld_hint<true>(&d.vbuf[c * cap + s]) : d.vbuf[c * cap + s];
+      const double v = vb + dt * 0.5 * (double)af[c];
       d.vel[c * cap + s] = v;
       if (vout) vout[c] = v;
     }

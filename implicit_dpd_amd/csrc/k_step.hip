@@ -285,6 +285,9 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
 // to its new slot.
 // tick: the step's clock advances here (no kernel of the step before the
 // force pass reads it)
+// ST: streaming loads of the old set and stores of the new set's position,
+// delta_sumr and ids (read again only by the next step)
+template <bool ST>
 __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin,
                                                  int drift, int tick) {
   if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
@@ -306,11 +309,11 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   for (int a = 0; a < 3; ++a) {
     // the old set's position, delta_sumr, velocity and acceleration are
     // read for the last time in this step: streaming loads
-    double x = __builtin_nontemporal_load(&d.pos[a * cap + i]);
-    double vb, dl = __builtin_nontemporal_load(&d.delta[a * cap + i]);
+    double x = ld_hint<ST>(&d.pos[a * cap + i]);
+    double vb, dl = ld_hint<ST>(&d.delta[a * cap + i]);
     if (drift) {
-      const DriftAxis o = drift_axis(R, a, x, __builtin_nontemporal_load(&d.vel[a * cap + i]),
-                                     (double)__builtin_nontemporal_load(&d.acc[a * cap + i]), dl);
+      const DriftAxis o = drift_axis(R, a, x, ld_hint<ST>(&d.vel[a * cap + i]),
+                                     (double)ld_hint<ST>(&d.acc[a * cap + i]), dl);
       x = o.p;
       dl = o.dl;
       vb = o.vb;
@@ -321,9 +324,9 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     // positions and delta_sumr are next read by the following step's
     // drift and permutation: streaming stores keep them from displacing
     // this step's pair-pass data in L2 (vel_buf is read by this step's kick)
-    __builtin_nontemporal_store(x, &d.alt.pos[a * cap + s]);
+    st_hint<ST>(&d.alt.pos[a * cap + s], x);
     d.alt.vbuf[a * cap + s] = vb;
-    __builtin_nontemporal_store(dl, &d.alt.delta[a * cap + s]);
+    st_hint<ST>(&d.alt.delta[a * cap + s], dl);
     // skin mode: the displacement test of the coming steps starts here
     if (skin) d.dref[a * cap + s] = dl;
     l[a] = (float)(x - (double)cc[a] * R.h[a]);
@@ -334,14 +337,14 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     }
     v[a] = (float)(predicted ? vb : d.vel[a * cap + i]);
   }
-  const uint32_t id = __builtin_nontemporal_load(&d.id[i]), pr = d.prop[i];
-  __builtin_nontemporal_store(id, &d.alt.id[s]);
+  const uint32_t id = ld_hint<ST>(&d.id[i]), pr = d.prop[i];
+  st_hint<ST>(&d.alt.id[s], id);
   d.alt.prop[s] = pr;
   d.alt.amp_id[s] = d.amp_id[i];
   d.alt.unit[s] = d.unit[i];
-  __builtin_nontemporal_store(__builtin_nontemporal_load(&d.orig[i]), &d.alt.orig[s]);
-  const uint32_t ch = __builtin_nontemporal_load(&d.chain[i]);
-  __builtin_nontemporal_store(ch, &d.alt.chain[s]);
+  st_hint<ST>(&d.alt.orig[s], ld_hint<ST>(&d.orig[i]));
+  const uint32_t ch = ld_hint<ST>(&d.chain[i]);
+  st_hint<ST>(&d.alt.chain[s], ch);
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(xs[0], xs[1], xs[2], 0.0);
   const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
   d.spos[s] = ps;
@@ -351,12 +354,15 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
 }
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
+  hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
                      full ? 1 : 0, 0, 0, 0);
 }
 void launch_step_permute(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
+  if (d.n >= kStreamMin)
+    hipLaunchKernelGGL(k_permute<true>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
+  else
+    hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
 }
 
 // chain bead slots follow the beads to their new slots
@@ -725,7 +731,7 @@ void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
 }
 void launch_permute_gated(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0, 0);
+  hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0, 0);
 }
 
 // The rebuild's permutation wrote the other buffer set; copy it back so the
