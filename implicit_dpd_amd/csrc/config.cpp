@@ -75,6 +75,10 @@ int plan_config(const mdpd_config_spec* s, Plan& P, std::string& err) {
     err = "null argument";
     return MDPD_ERR_ARG;
   }
+  if ((uint64_t)s->head_unit + s->tail_unit > kIndexMask) {
+    err = "head_unit + tail_unit is too large";
+    return MDPD_ERR_ARG;
+  }
   const uint32_t all_unit = s->head_unit + s->tail_unit;
   const uint64_t amp_beads = (uint64_t)all_unit * s->amp_num;
   const uint64_t n = amp_beads + (s->with_solvent ? s->sol_num : 0u);
@@ -404,7 +408,18 @@ int mdpd_make_config(mdpd_ctx* c, const mdpd_config_spec* s, double* kin_temp) {
   const size_t cap = d.cap;
   CfgOut o{d.pos, d.vel, d.vbuf, d.id, d.prop, d.amp_id, d.unit, 1, cap};
   rc = generate(P, o, c->st, kin_temp, err);
-  if (rc) return fail(c, rc, err);
+  if (rc) {
+    // the kernels may have overwritten the old beads: leave no beads rather
+    // than a mix (the reference's config_maker exits here)
+    invalidate_graphs(c);
+    d.n = 0;
+    d.n_amp = 0;
+    c->n_amp = 0;
+    c->rep_off[1] = 0;
+    c->R[0].n_beads = 0;
+    c->have_cells = c->cells_fresh = c->have_density = false;
+    return fail(c, rc, err);
+  }
   const uint32_t n = P.n;
   for (int a = 0; a < 3; ++a) {
     HIPCHK(c, hipMemsetAsync(d.delta + a * cap, 0, 8 * (size_t)n, c->st));

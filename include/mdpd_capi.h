@@ -371,7 +371,8 @@ int mdpd_make_config_host(const mdpd_config_spec* s, uint32_t n, double* pos,
    become the context's state and, for 4-bead amphiphiles (head_unit +
    tail_unit == 4, the engine's chain model), the chains its topology with
    no core amphiphiles (mdpd_set_topology may follow).  Other chain lengths
-   are refused with MDPD_ERR_UNSUPPORTED. */
+   are refused with MDPD_ERR_UNSUPPORTED.  A configuration that fails the
+   checks above leaves the context with no beads. */
 int mdpd_make_config(mdpd_ctx* ctx, const mdpd_config_spec* s, double* kin_temp);
 
 /* ---- equation-of-state surrogate (no reference counterpart) ------------

@@ -120,6 +120,11 @@ def test_make_config_into_engine_and_run(engine_mod, oracle):
     with pytest.raises(MdpdError) as ei:
         e.make_config(cm.spec("random", box, 100, head_unit=2, tail_unit=3))
     assert ei.value.code == MDPD_ERR_UNSUPPORTED
+    # a spec that fails config_maker's checks (16 beads: |kT - 1| >= 1e-2)
+    # leaves no beads behind rather than a mix of old and new ones
+    with pytest.raises(MdpdError) as ei:
+        e.make_config(cm.spec("random", box, 4, seed=1))
+    assert ei.value.code == MDPD_ERR_CONFIG and e.num_beads() == 0
     e.close()
 
 
