@@ -195,6 +195,8 @@ PROTOTYPES = {
     "mdpd_observe_replicas": (C.c_int, [VP, P(MdpdObservables)]),
     "mdpd_density_profile": (C.c_int, [VP, C.c_int32, C.c_int32, _DP]),
     "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
+    "mdpd_write_xyz_async": (C.c_int, [VP, C.c_char_p, C.c_uint32, C.c_int32]),
+    "mdpd_io_wait": (C.c_int, [VP]),
     "mdpd_checkpoint_save": (C.c_int, [VP, C.c_char_p]),
     "mdpd_checkpoint_load": (C.c_int, [VP, C.c_char_p]),
     "mdpd_chain_histograms": (C.c_int, [VP, C.c_int32, C.c_double, _DP]),

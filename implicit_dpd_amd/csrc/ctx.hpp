@@ -11,6 +11,8 @@
 
 namespace mdpd {
 extern thread_local std::string g_err;
+struct XyzWriter;  // asynchronous trajectory output (io.cpp)
+void xyz_writer_free(XyzWriter* w);
 constexpr int kPhases = 10;
 extern const char* kPhaseNames[kPhases];
 }  // namespace mdpd
@@ -52,6 +54,7 @@ struct mdpd_ctx {
   double ph_ms[mdpd::kPhases + 1] = {0};  // accumulated; last = whole step
   uint64_t ph_steps = 0;
   std::vector<void*> allocs;
+  mdpd::XyzWriter* xyz = nullptr;
 };
 
 namespace mdpd {

@@ -470,6 +470,7 @@ int mdpd_create_ensemble(const mdpd_params* p, const uint32_t* n_beads, int32_t 
 
 int mdpd_destroy(mdpd_ctx* c) {
   if (!c) return MDPD_OK;
+  xyz_writer_free(c->xyz);  // finishes a trajectory write in flight
   if (c->st) hipStreamSynchronize(c->st);
   invalidate_graphs(c);
   for (auto& e : c->evpool) hipEventDestroy(e);

@@ -989,4 +989,30 @@ void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_d, dim3(1024), dim3(256), 0, st, dst, src, n);
 }
 
+// The 17 columns of FPDPD::writeAscii (ptcl_class.hpp:67-73) of the bead in
+// storage slot i, at row orig[i] (its upload slot) of out.
+__global__ void __launch_bounds__(256) k_xyz_gather(DevState d, const double* __restrict__ vel,
+                                                    XyzRec* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  XyzRec r;
+  const size_t cap = d.cap;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    r.pos[a] = d.pos[a * cap + i];
+    r.vel[a] = vel[a * cap + i];
+    r.vbuf[a] = d.vbuf[a * cap + i];
+    r.acc[a] = d.acc[a * cap + i];
+  }
+  r.id = d.id[i];
+  r.prop = d.prop[i];
+  r.amp_id = d.amp_id[i];
+  r.unit = d.unit[i];
+  out[d.orig[i]] = r;
+}
+void launch_xyz_gather(const DevState& d, const double* vel, XyzRec* out, hipStream_t st) {
+  if (!d.n) return;
+  hipLaunchKernelGGL(k_xyz_gather, grid1(d.n, 256), dim3(256), 0, st, d, vel, out);
+}
+
 }  // namespace mdpd

@@ -162,5 +162,12 @@ void launch_chem_flags(const DevState& d, const double* core /*[n_core][6]: pos 
 void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget,
                         double eps, double bond_leng, uint32_t time, hipStream_t st);
 void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st);
+// one bead's columns of the xyz trajectory format, in upload order
+struct XyzRec {
+  double pos[3], vel[3], vbuf[3];
+  float acc[3];
+  uint32_t id, prop, amp_id, unit;
+};
+void launch_xyz_gather(const DevState& d, const double* vel, XyzRec* out, hipStream_t st);
 
 }  // namespace mdpd

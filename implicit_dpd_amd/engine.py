@@ -226,6 +226,14 @@ class Engine:
         self._c(self.L.mdpd_density_profile(self.h, axis, nbins, _d(out)))
         return out
 
+    def write_xyz_async(self, path, time, append=False):
+        """Trajectory frame in the reference's xyz format, written by a host
+        thread while the caller goes on; io_wait() waits for it."""
+        self._c(self.L.mdpd_write_xyz_async(self.h, str(path).encode(), int(time), 1 if append else 0))
+
+    def io_wait(self):
+        self._c(self.L.mdpd_io_wait(self.h))
+
     def checkpoint_save(self, path):
         self._c(self.L.mdpd_checkpoint_save(self.h, str(path).encode()))
 

@@ -210,6 +210,20 @@ int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
 int mdpd_observe(mdpd_ctx* ctx, mdpd_observables* out);
 /* observables of every replica of an ensemble context, out[n_rep] */
 int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
+/* Trajectory / configuration output in the reference's xyz text format
+   (io_util::WriteXYZForm + FPDPD::writeAscii, io_util.hpp:25-31,
+   ptcl_class.hpp:67-73; the writer of traject.xyz every step_mic and of
+   fin_config.xyz, main.cpp:199-208): count line, "time T", one 17-column
+   line per bead in upload order, reals as %.15g.  The state is copied into
+   pinned host memory on the context's stream and the call returns; a host
+   thread formats (in parallel) and writes the file while the caller goes on
+   (append != 0: add a frame).  The next write, mdpd_io_wait and
+   mdpd_destroy wait for it; mdpd_io_wait returns the status of the write
+   (file errors as MDPD_ERR_ARG).  No reference counterpart for the
+   asynchrony: the reference formats on the calling thread. */
+int mdpd_write_xyz_async(mdpd_ctx* ctx, const char* path, uint32_t time, int32_t append);
+int mdpd_io_wait(mdpd_ctx* ctx);
+
 /* Binary checkpoint of the whole evolving state in the device's storage
    order (positions, velocities, vel_buf, delta_sumr, accelerations, virials,
    ids and types, chain map and topology, clock and pair seeds).  Loading it

@@ -296,6 +296,12 @@ class Observer {
       fp_[i] = std::fopen(f.c_str(), "w");
       if (!fp_[i]) Abort("cannot open " + f);
     }
+    // the engine's writer appends the trajectory frames and writes the final
+    // configuration itself: these two stay empty files until it does
+    for (int i : {kTraj, kFin}) {
+      std::fclose(fp_[i]);
+      fp_[i] = nullptr;
+    }
   }
   void KineticTempera(ParticleSystem& s) {
     const mdpd_observables& o = Observe(s);
@@ -327,8 +333,17 @@ class Observer {
                  o.diffusion[2]);
   }
   void NumAmp(uint32_t amp_num) { std::fprintf(fp_[kNumAmp], "%u\n", amp_num); }
-  void Trajectory(ParticleSystem& s) { WriteXYZForm(s.getParticles(), Clock::time, fp_[kTraj]); }
-  void FinConfig(ParticleSystem& s) { WriteXYZForm(s.getParticles(), Clock::time, fp_[kFin]); }
+  // a frame of traject.xyz, formatted and written by a host thread while the
+  // run goes on (mdpd_write_xyz_async)
+  void Trajectory(ParticleSystem& s) {
+    Check(mdpd_write_xyz_async(s.ctx(), (cdir_ + "/traject.xyz").c_str(), Clock::time, 1), s.ctx(),
+          "mdpd_write_xyz_async");
+  }
+  void FinConfig(ParticleSystem& s) {
+    Check(mdpd_write_xyz_async(s.ctx(), (cdir_ + "/fin_config.xyz").c_str(), Clock::time, 0), s.ctx(),
+          "mdpd_write_xyz_async");
+    Check(mdpd_io_wait(s.ctx()), s.ctx(), "mdpd_io_wait");
+  }
   void FlushAll() {
     for (FILE* f : fp_)
       if (f) std::fflush(f);
