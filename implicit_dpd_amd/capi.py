@@ -197,6 +197,8 @@ PROTOTYPES = {
     "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
     "mdpd_write_xyz_async": (C.c_int, [VP, C.c_char_p, C.c_uint32, C.c_int32]),
     "mdpd_io_wait": (C.c_int, [VP]),
+    "mdpd_read_xyz_header": (C.c_int, [C.c_char_p, _U32P, _U32P]),
+    "mdpd_read_xyz": (C.c_int, [C.c_char_p, VP, C.c_uint32, C.c_size_t, _U32P, _U32P]),
     "mdpd_checkpoint_save": (C.c_int, [VP, C.c_char_p]),
     "mdpd_checkpoint_load": (C.c_int, [VP, C.c_char_p]),
     "mdpd_chain_histograms": (C.c_int, [VP, C.c_int32, C.c_double, _DP]),

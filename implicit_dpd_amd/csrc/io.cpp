@@ -12,7 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -143,6 +146,169 @@ int mdpd_write_xyz_async(mdpd_ctx* c, const char* path, uint32_t time, int32_t a
 int mdpd_io_wait(mdpd_ctx* c) {
   if (!c) return MDPD_ERR_ARG;
   return join(c);
+}
+
+// ---------------------------------------------------------------- reading
+// io_util::ReadXYZForm (io_util.hpp:103-135) with FPDPD::readAscii
+// (ptcl_class.hpp:60-65): "%u" count line, "time %u" line, then one line of
+// "%c %lf %lf %lf %u %u %u %u %lf x9" per bead; the other fields of the
+// record (delta_sumr, press, nei_cm_pos, density) are zero.  The file is
+// read once; line starts are found and lines parsed in parallel chunks with
+// strtod / strtoul (the conversions of fscanf's %lf / %u).
+
+static int xyz_header(const char* b, const char* e, uint32_t* n, uint32_t* time, const char** body,
+                      std::string& err) {
+  // "%u\n" then "%s %u\n"
+  char* q = nullptr;
+  const char* p = b;
+  while (p < e && std::isspace((unsigned char)*p)) ++p;
+  const unsigned long nv = std::strtoul(p, &q, 10);
+  if (q == p) {
+    err = "cannot read the number of particles";
+    return MDPD_ERR_PARSE;
+  }
+  p = q;
+  while (p < e && std::isspace((unsigned char)*p)) ++p;
+  const char* tag = p;
+  while (p < e && !std::isspace((unsigned char)*p)) ++p;
+  if (std::string(tag, p) != "time") {
+    err = "The file format of init_config.xyz may be old.";
+    return MDPD_ERR_PARSE;
+  }
+  const unsigned long tv = std::strtoul(p, &q, 10);
+  if (q == p || nv > 0xffffffffUL || tv > 0xffffffffUL) {
+    err = "The file format of init_config.xyz may be old.";
+    return MDPD_ERR_PARSE;
+  }
+  p = q;
+  while (p < e && *p != '\n') ++p;
+  if (p < e) ++p;
+  *n = (uint32_t)nv;
+  *time = (uint32_t)tv;
+  *body = p;
+  return MDPD_OK;
+}
+
+// one record line [p, e); false if it does not hold the 17 columns
+static bool xyz_line(const char* p, const char* e, char* rec) {
+  while (p < e && (*p == ' ' || *p == '\t')) ++p;
+  if (p >= e) return false;
+  ++p;  // the type letter (%c)
+  double v[12];
+  uint32_t u[4];
+  char* q = nullptr;
+  for (int k = 0; k < 3; ++k) {
+    v[k] = std::strtod(p, &q);
+    if (q == p || q > e) return false;  // a short line must not run into the next
+    p = q;
+  }
+  for (int k = 0; k < 4; ++k) {
+    const unsigned long x = std::strtoul(p, &q, 10);
+    if (q == p || q > e) return false;
+    u[k] = (uint32_t)x;
+    p = q;
+  }
+  for (int k = 3; k < 12; ++k) {
+    v[k] = std::strtod(p, &q);
+    if (q == p || q > e) return false;
+    p = q;
+  }
+  std::memset(rec, 0, MDPD_FPDPD_STRIDE);
+  std::memcpy(rec + 0, u, 16);          // id, prop, amp_id, unit
+  std::memcpy(rec + 16, v, 24);         // pos
+  std::memcpy(rec + 64, v + 3, 72);     // vel, vel_buf, acc
+  return true;
+}
+
+int mdpd_read_xyz_header(const char* path, uint32_t* n, uint32_t* time) {
+  if (!path || !n || !time) return fail(nullptr, MDPD_ERR_ARG, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(nullptr, MDPD_ERR_PARSE, std::string("cannot open ") + path);
+  char buf[256];
+  const size_t k = std::fread(buf, 1, sizeof(buf), f);
+  std::fclose(f);
+  const char* body = nullptr;
+  std::string err;
+  const int rc = xyz_header(buf, buf + k, n, time, &body, err);
+  if (rc) return fail(nullptr, rc, err);
+  return MDPD_OK;
+}
+
+int mdpd_read_xyz(const char* path, void* aos, uint32_t cap, size_t stride, uint32_t* n_out,
+                  uint32_t* time_out) {
+  if (!path || (!aos && cap) || !n_out || !time_out) return fail(nullptr, MDPD_ERR_ARG, "null argument");
+  if (stride < MDPD_FPDPD_STRIDE) return fail(nullptr, MDPD_ERR_ARG, "stride < 232");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(nullptr, MDPD_ERR_PARSE, std::string("cannot open ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  std::vector<char> data((size_t)std::max(sz, 0L) + 1, '\0');
+  const size_t got = std::fread(data.data(), 1, (size_t)std::max(sz, 0L), f);
+  std::fclose(f);
+  const char* b = data.data();
+  const char* e = b + got;
+  uint32_t n = 0, time = 0;
+  const char* body = nullptr;
+  std::string err;
+  int rc = xyz_header(b, e, &n, &time, &body, err);
+  if (rc) return fail(nullptr, rc, err);
+  if (n > cap) return fail(nullptr, MDPD_ERR_CAPACITY,
+                           "# of particles specified in init_config.xyz exceeds maximum buffer.");
+  // line starts, in parallel chunks (blank lines are not records)
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t len = (size_t)(e - body);
+  const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>({16u, hw, len / (1u << 20) + 1}));
+  std::vector<std::vector<const char*>> starts(nt);
+  {
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < nt; ++k)
+      pool.emplace_back([&, k] {
+        const char* lo = body + len * k / nt;
+        const char* hi = body + len * (k + 1) / nt;
+        // a line belongs to the chunk holding its first character
+        const char* p = lo;
+        if (k > 0 && p[-1] != '\n') {
+          while (p < hi && *p != '\n') ++p;
+          if (p < hi) ++p;
+        }
+        while (p < hi) {
+          const char* q = p;
+          while (q < e && *q != '\n') ++q;
+          bool blank = true;
+          for (const char* r = p; r < q; ++r)
+            if (!std::isspace((unsigned char)*r)) { blank = false; break; }
+          if (!blank) starts[k].push_back(p);
+          p = q < e ? q + 1 : e;
+        }
+      });
+    for (auto& t : pool) t.join();
+  }
+  size_t lines = 0;
+  for (auto& v : starts) lines += v.size();
+  if (lines != n)
+    return fail(nullptr, MDPD_ERR_PARSE, "# of lines is not equal to the information specified in file header.");
+  std::vector<size_t> first(nt + 1, 0);
+  for (unsigned k = 0; k < nt; ++k) first[k + 1] = first[k] + starts[k].size();
+  std::vector<int> bad(nt, 0);
+  {
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < nt; ++k)
+      pool.emplace_back([&, k] {
+        for (size_t j = 0; j < starts[k].size(); ++j) {
+          const char* p = starts[k][j];
+          const char* q = p;
+          while (q < e && *q != '\n') ++q;
+          if (!xyz_line(p, q, (char*)aos + (first[k] + j) * stride)) bad[k] = 1;
+        }
+      });
+    for (auto& t : pool) t.join();
+  }
+  for (int x : bad)
+    if (x) return fail(nullptr, MDPD_ERR_PARSE, "a particle line does not hold the 17 columns");
+  *n_out = n;
+  *time_out = time;
+  return MDPD_OK;
 }
 
 }  // extern "C"

@@ -317,3 +317,17 @@ def device_name():
     b = C.create_string_buffer(256)
     check(capi.lib().mdpd_device_name(b, 256))
     return b.value.decode()
+
+
+def read_xyz(path):
+    """io_util::ReadXYZForm through the library's parallel reader: returns
+    (FPDPD records, time).  Raises MdpdError on the reference's abort cases."""
+    from .fpdpd import FPDPD_DTYPE
+    L = capi.lib()
+    n, t = C.c_uint32(), C.c_uint32()
+    p = str(path).encode()
+    check(L.mdpd_read_xyz_header(p, C.byref(n), C.byref(t)))
+    rec = np.zeros(n.value, dtype=FPDPD_DTYPE)
+    buf = rec.ctypes.data_as(C.c_void_p) if n.value else None
+    check(L.mdpd_read_xyz(p, buf, n.value, FPDPD_DTYPE.itemsize, C.byref(n), C.byref(t)))
+    return rec, t.value

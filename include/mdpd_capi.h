@@ -223,6 +223,19 @@ int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
    asynchrony: the reference formats on the calling thread. */
 int mdpd_write_xyz_async(mdpd_ctx* ctx, const char* path, uint32_t time, int32_t append);
 int mdpd_io_wait(mdpd_ctx* ctx);
+/* Reads a configuration in the reference's xyz text format into FPDPD
+   records `stride` bytes apart (io_util::ReadXYZForm + FPDPD::readAscii,
+   io_util.hpp:96-129, ptcl_class.hpp:60-65; the loader of init_config.xyz,
+   Parameter::LoadParticleConfig, parameter.hpp:420-428).  Host only, no
+   context: the file is read once and its lines parsed in parallel.  The
+   header must be "<n>" then "time <t>"; more than `cap` records gives
+   MDPD_ERR_CAPACITY, a line count other than <n> or a line without its 17
+   columns MDPD_ERR_PARSE (the reference aborts in these cases); the message
+   is in mdpd_global_error.  delta_sumr, press, nei_cm_pos and density are
+   zero.  mdpd_read_xyz_header reads only <n> and <t>, to size the records. */
+int mdpd_read_xyz_header(const char* path, uint32_t* n, uint32_t* time);
+int mdpd_read_xyz(const char* path, void* aos, uint32_t cap, size_t stride, uint32_t* n,
+                  uint32_t* time);
 
 /* Binary checkpoint of the whole evolving state in the device's storage
    order (positions, velocities, vel_buf, delta_sumr, accelerations, virials,

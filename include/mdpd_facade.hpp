@@ -196,25 +196,13 @@ inline void WriteXYZForm(const std::vector<FPDPD>& v, uint32_t time, FILE* fp) {
 }
 
 // io_util::ReadXYZForm + FPDPD::readAscii; returns the time of the header.
+// The library reads the file once and parses its lines in parallel.
 inline uint32_t ReadXYZForm(const std::string& fname, std::vector<FPDPD>& v) {
-  FILE* fp = std::fopen(fname.c_str(), "r");
-  if (!fp) Abort("cannot open " + fname);
-  unsigned n = 0, t = 0;
-  char tag[16] = {0};
-  if (std::fscanf(fp, "%u\n", &n) != 1 || std::fscanf(fp, "%15s %u\n", tag, &t) != 2 ||
-      std::string(tag) != "time")
-    Abort("The file format of " + fname + " may be old.");
+  uint32_t n = 0, t = 0;
+  if (mdpd_read_xyz_header(fname.c_str(), &n, &t) != MDPD_OK) Abort(mdpd_global_error());
   v.assign(n, FPDPD{});
-  for (unsigned i = 0; i < n; ++i) {
-    FPDPD& p = v[i];
-    char c;
-    if (std::fscanf(fp, " %c %lf %lf %lf %u %u %u %u %lf %lf %lf %lf %lf %lf %lf %lf %lf", &c,
-                    &p.pos[0], &p.pos[1], &p.pos[2], &p.id, &p.prop, &p.amp_id, &p.unit,
-                    &p.vel[0], &p.vel[1], &p.vel[2], &p.vel_buf[0], &p.vel_buf[1],
-                    &p.vel_buf[2], &p.acc[0], &p.acc[1], &p.acc[2]) != 17)
-      Abort("# of lines is not equal to the information specified in file header.");
-  }
-  std::fclose(fp);
+  if (mdpd_read_xyz(fname.c_str(), v.data(), n, sizeof(FPDPD), &n, &t) != MDPD_OK)
+    Abort(mdpd_global_error());
   return t;
 }
 
