@@ -393,7 +393,7 @@ def main():
     dens_algo = (skin_density_bytes_per_launch(n, n_pairs, nd, n_skin) if args.skin > 0
                  else density_bytes_per_launch(n, n_pairs, nd, w.ncell))
     for name, phase, algo in (
-            ("k_density", "density", dens_algo),
+            ("k_density_skin" if args.skin > 0 else "k_density_q", "density", dens_algo),
             ("k_force", "force", force_bytes_per_launch(n, n_pairs, nd))):
         ms = ph[phase]
         gbs = algo / (ms * 1e-3) / 1e9

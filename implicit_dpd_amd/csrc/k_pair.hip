@@ -72,42 +72,32 @@ __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s)
          (s & 63u);
 }
 
-// BUILD: the skin-list build of skin mode instead (rebuild steps only):
-// every bead but the bead itself with FP32 r^2 < vcut2 goes to the skin
-// list; vcut2 carries a margin over (1 + skin)^2 that absorbs the FP32
-// error, and cells are at least 1 + skin wide, so no pair that can come
-// inside the cutoff before the next rebuild is left out.
-// RDLE1: every rd <= 1, so a pair strictly inside cut_hi and outside the
-// guard bands is inside the cutoff 1 (no compare needed).
-template <int ND, bool MULTI, bool BUILD, bool RDLE1>
-__global__ void __launch_bounds__(256) k_density(DevState d) {
-  if (BUILD && *d.rebuild == 0u) return;
+// Skin-list build of skin mode (rebuild steps only): every bead but the
+// bead itself with FP32 r^2 < vcut2 goes to the bead's skin list, written
+// like the pair list ([tile][chunk][lane] chunks of four entries).  vcut2
+// carries a margin over (1 + skin)^2 that absorbs the FP32 error, and cells
+// are at least 1 + skin wide, so no pair that can come inside the cutoff
+// before the next rebuild is left out.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_skin_build(DevState d) {
+  if (*d.rebuild == 0u) return;
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
-  // replica constants in registers for the loops below
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
   const uint32_t cell_off = R.cell_off;
   const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
-  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
-  const float cut_hi = BUILD ? d.vcut2 : fmaxf(1.0f, rd2f) + kGuard;
+  const float cut_hi = d.vcut2;
   const float4 pi = d.spos[s];
   const uint32_t c = d.scell[s] - cell_off;
   const int cx = (int)(c % (uint32_t)nx);
   const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
   const int cz = (int)(c / (uint32_t)(nx * ny));
-  const uint32_t W = BUILD ? (uint32_t)d.Wv : (uint32_t)d.W;
-  uint4* lst = BUILD ? reinterpret_cast<uint4*>(d.vlist) + (size_t)(s >> 6) * (W >> 2) * 64u + (s & 63u)
-                     : const_cast<uint4*>(tile_list(d, s));
-  float rho[ND];
-#pragma unroll
-  for (int k = 0; k < ND; ++k) rho[k] = 0.f;
-  // The bead's x offset term for the left, own and right neighbour cell;
-  // a candidate picks one by comparing its slot with two run boundaries.
+  const uint32_t W = (uint32_t)d.Wv;
+  uint4* lst = reinterpret_cast<uint4*>(d.vlist) + (size_t)(s >> 6) * (W >> 2) * 64u + (s & 63u);
+  // the bead's x offset term for the left, own and right neighbour cell
   const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
-  // Pair-list entries are buffered in a three-deep shift register and
-  // written as whole 16-byte chunks of four.
   uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
   const bool wrap = !(cx >= 1 && cx <= nx - 2);
   const uint32_t kInf = 0xFFFFFFFFu;
@@ -121,17 +111,10 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
       qy += (qy < 0) ? ny : 0;
       qy -= (qy >= ny) ? ny : 0;
       const float by = pi.y - (float)oy * hy;
-      // The three x-neighbour cells of a row are consecutive in cell order,
-      // hence one contiguous run of slots (two runs where the row wraps).
-      const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
-      const uint32_t* S = d.cell_start + rowq;
+      const uint32_t* S = d.cell_start + cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      // per-lane registers: a pair-list entry then selects its code without
-      // first copying the three run constants into vector registers
-      uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
-      asm volatile("" : "+v"(C0), "+v"(C1), "+v"(C2));
+      const uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
       for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
-        // run [ta, tb); slots >= b2 are in the right cell, >= b1 the own cell
         uint32_t ta, tb, b1, b2;
         if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
         else if (cx == 0) {
@@ -142,72 +125,21 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
           else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
         }
         for (uint32_t t = ta; t < tb; ++t) {
-          // (other waves hide the load latency; a software prefetch costs
-          // six register moves per candidate)
           const float4 pj = d.spos[t];
           const bool g2 = t >= b2, g1 = t >= b1;
           const float bx = g2 ? X2 : (g1 ? X1 : X0);
           const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
-          float r2 = dx * dx + dy * dy + dz * dz;
-          if (BUILD) {
-            if (r2 >= cut_hi || t == s) continue;
-            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
-            if ((cnt & 3u) == 3u) {
-              if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
-            } else {
-              e0 = e1;
-              e1 = e2;
-              e2 = e;
-            }
-            ++cnt;
-            continue;
-          }
-          if (r2 >= cut_hi) continue;
-          // Outside the guard bands the FP32 r^2 decides, and there the
-          // clamp of w at 0 is the rd test (r^2 > rd^2 + kGuard gives
-          // 1 - r/rd < 0 far beyond the FP32 error), so the common path
-          // has no rd compare; with RDLE1 it has no cutoff compare either.
-          bool in_rc = RDLE1 ? true : (r2 < 1.0f);
-          // next to a cutoff, or (near-)coincident: decide in FP64; the
-          // bead itself (t == s, r2 == 0) lands here and is dropped
-          const bool near = (fabsf(r2 - 1.0f) <= kGuard) | (fabsf(r2 - rd2f) <= kGuard) |
-                            (r2 < 1e-6f);
-          if (near) {
-            if (t == s) continue;
-            // opaque copy: keeps the compiler from stepping an FP64 address
-            // along with t in every iteration for this rare path
-            uint32_t tt = t;
-            asm volatile("" : "+v"(tt));
-            const double r2d = fp64_r2(d, R, s, tt);
-            in_rc = r2d < 1.0;
-            // outside rd (or coincident): a value that makes w = 0
-            if (!(r2d < R.rd2 && r2d != 0.0)) r2 = 2.0f * rd2f + 1.0f;
-          }
-          {
-            const float rr = __builtin_amdgcn_sqrtf(r2);
-            const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
-            const float v = crho * w * w;
-            if (ND == 1) rho[0] += v;
-            else {
-              const uint32_t tj = __float_as_uint(pj.w) & 3u;
-#pragma unroll
-              for (int k = 0; k < ND; ++k) rho[k] += (tj == (uint32_t)k) ? v : 0.f;
-            }
-          }
-          if (in_rc) {
-            // the register window always shifts; every fourth entry the
-            // window plus the entry that fell out go out as one chunk
-            const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
-            const uint32_t out = e0;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 >= cut_hi || t == s) continue;
+          const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
+          if ((cnt & 3u) == 3u) {
+            if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
+          } else {
             e0 = e1;
             e1 = e2;
             e2 = e;
-            // past W entries the last chunk is overwritten; the overflow
-            // flag below fails the call
-            if ((cnt & 3u) == 3u)
-              lst[(size_t)min(cnt >> 2, (W >> 2) - 1u) * 64u] = make_uint4(out, e0, e1, e2);
-            ++cnt;
           }
+          ++cnt;
         }
       }
     }
@@ -220,39 +152,184 @@ __global__ void __launch_bounds__(256) k_density(DevState d) {
                                 : make_uint4(e0, e1, e2, 0u);
     lst[(size_t)(cnt >> 2) * 64u] = ch;
   }
-  if (BUILD) {
-    d.vcount[s] = cnt;
-  } else {
+  d.vcount[s] = cnt;
+  if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
+}
+
+// Density pass, queued form (the every-step pass).  The candidate scan
+// only forms the FP32 r^2; a candidate inside cut_hi goes into the lane's
+// queue in LDS as (entry, r^2), kDensQ deep.  Once some lane of the wave has
+// filled its queue, and after the scan, every active lane drains its own
+// queue in order: the guard-band decisions, the density weight and the
+// pair-list write run there with most lanes busy, rather than inside the
+// scan with only the lanes whose candidate is in range (about one in six).
+// Each lane still meets its in-range candidates in scan order, so the FP32
+// sums and the pair list are those of a plain per-candidate loop.
+constexpr uint32_t kDensQ = 8;
+
+template <int ND, bool MULTI, bool RDLE1>
+__global__ void __launch_bounds__(256) k_density_q(DevState d) {
+  extern __shared__ uint2 dens_queue[];  // [kDensQ][blockDim.x]
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+  const uint32_t cell_off = R.cell_off;
+  const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
+  const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
+  const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
+  const float4 pi = d.spos[s];
+  const uint32_t c = d.scell[s] - cell_off;
+  const int cx = (int)(c % (uint32_t)nx);
+  const int cy = (int)((c / (uint32_t)nx) % (uint32_t)ny);
+  const int cz = (int)(c / (uint32_t)(nx * ny));
+  const uint32_t W = (uint32_t)d.W;
+  uint4* lst = const_cast<uint4*>(tile_list(d, s));
+  const uint32_t qs = blockDim.x, qi = threadIdx.x;
+  uint32_t nq = 0;
+  float rho[ND];
 #pragma unroll
-    for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
-    // one density: also in the velocity copy's spare lane, so the force
-    // pass gets a neighbour's velocity and density in one 16-byte load
-    if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
-    d.ncount[s] = cnt;
+  for (int k = 0; k < ND; ++k) rho[k] = 0.f;
+  uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
+  // the queued candidates, in order: decisions, weights, pair-list entries
+  auto drain = [&]() {
+    for (uint32_t k = 0; k < nq; ++k) {
+      const uint2 en = dens_queue[qi + k * qs];
+      const uint32_t e = en.x, t = e & kIndexMask;
+      float r2 = __uint_as_float(en.y);
+      // outside the guard bands the FP32 r^2 decides; the clamp of w at 0
+      // is then the rd test, and with RDLE1 no cutoff compare is needed
+      bool in_rc = RDLE1 ? true : (r2 < 1.0f);
+      const bool near = (fabsf(r2 - 1.0f) <= kGuard) | (fabsf(r2 - rd2f) <= kGuard) |
+                        (r2 < 1e-6f);
+      if (near) {
+        if (t == s) continue;  // the bead itself
+        const double r2d = fp64_r2(d, R, s, t);
+        in_rc = r2d < 1.0;
+        if (!(r2d < R.rd2 && r2d != 0.0)) r2 = 2.0f * rd2f + 1.0f;
+      }
+      const float rr = __builtin_amdgcn_sqrtf(r2);
+      const float w = fmaxf(1.0f - rr * inv_rd, 0.0f);
+      const float v = crho * w * w;
+      if (ND == 1) rho[0] += v;
+      else {
+        const uint32_t tj = __float_as_uint(d.spos[t].w) & 3u;
+#pragma unroll
+        for (int k2 = 0; k2 < ND; ++k2) rho[k2] += (tj == (uint32_t)k2) ? v : 0.f;
+      }
+      if (in_rc) {
+        // three-deep shift register; every fourth entry the window plus the
+        // entry that fell out go out as one 16-byte chunk (past W entries
+        // the last chunk is overwritten and the overflow flag fails the call)
+        const uint32_t out = e0;
+        e0 = e1;
+        e1 = e2;
+        e2 = e;
+        if ((cnt & 3u) == 3u)
+          lst[(size_t)min(cnt >> 2, (W >> 2) - 1u) * 64u] = make_uint4(out, e0, e1, e2);
+        ++cnt;
+      }
+    }
+    nq = 0;
+  };
+  const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
+  const bool wrap = !(cx >= 1 && cx <= nx - 2);
+  const uint32_t kInf = 0xFFFFFFFFu;
+  for (int oz = -1; oz <= 1; ++oz) {
+    int qz = cz + oz;
+    qz += (qz < 0) ? nz : 0;
+    qz -= (qz >= nz) ? nz : 0;
+    const float bz = pi.z - (float)oz * hz;
+    for (int oy = -1; oy <= 1; ++oy) {
+      int qy = cy + oy;
+      qy += (qy < 0) ? ny : 0;
+      qy -= (qy >= ny) ? ny : 0;
+      const float by = pi.y - (float)oy * hy;
+      const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
+      const uint32_t* S = d.cell_start + rowq;
+      const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
+      uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
+      asm volatile("" : "+v"(C0), "+v"(C1), "+v"(C2));
+      for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
+        // run [ta, tb) over the row's three cells; slots >= b2 are in the
+        // right cell, >= b1 the own cell
+        uint32_t ta, tb, b1, b2;
+        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
+        else if (cx == 0) {
+          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; b1 = kInf; b2 = kInf; }
+          else { ta = S[0]; tb = S[2]; b1 = 0; b2 = S[1]; }
+        } else {
+          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
+          else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
+        }
+        // two candidates per iteration, two gathers in flight per wave; the
+        // second of an odd run's last pair is a repeated load, not queued.
+        // The queue keeps room for both (drained when fewer than two are free).
+        for (uint32_t t = ta; t < tb; t += 2u) {
+          const uint32_t u = min(t + 1u, tb - 1u);
+          const float4 pa = d.spos[t], pb = d.spos[u];
+          {
+            const bool g2 = t >= b2, g1 = t >= b1;
+            const float bx = g2 ? X2 : (g1 ? X1 : X0);
+            const float dx = bx - pa.x, dy = by - pa.y, dz = bz - pa.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < cut_hi) {
+              dens_queue[qi + nq * qs] = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+              ++nq;
+            }
+          }
+          {
+            const bool g2 = u >= b2, g1 = u >= b1;
+            const float bx = g2 ? X2 : (g1 ? X1 : X0);
+            const float dx = bx - pb.x, dy = by - pb.y, dz = bz - pb.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < cut_hi && u != t) {
+              dens_queue[qi + nq * qs] = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+              ++nq;
+            }
+          }
+          if (__ballot(nq + 2u > kDensQ) != 0ull) drain();
+        }
+      }
+    }
   }
+  drain();
+  const uint32_t tail = cnt & 3u;
+  if (tail != 0u && cnt < W) {
+    const uint4 ch = tail == 1u ? make_uint4(e2, 0u, 0u, 0u)
+                   : tail == 2u ? make_uint4(e1, e2, 0u, 0u)
+                                : make_uint4(e0, e1, e2, 0u);
+    lst[(size_t)(cnt >> 2) * 64u] = ch;
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+  if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
+  d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
 }
 
 template <int ND, bool MULTI>
 static void dens_launch(const DevState& d, int block, hipStream_t st) {
+  const size_t lds = (size_t)kDensQ * (size_t)block * sizeof(uint2);
   if (d.rd_le_1)
-    hipLaunchKernelGGL((k_density<ND, MULTI, false, true>), dim3(div_up(d.n, block)), dim3(block), 0,
+    hipLaunchKernelGGL((k_density_q<ND, MULTI, true>), dim3(div_up(d.n, block)), dim3(block), lds,
                        st, d);
   else
-    hipLaunchKernelGGL((k_density<ND, MULTI, false, false>), dim3(div_up(d.n, block)), dim3(block),
-                       0, st, d);
+    hipLaunchKernelGGL((k_density_q<ND, MULTI, false>), dim3(div_up(d.n, block)), dim3(block), lds,
+                       st, d);
 }
 
 void launch_skin_build(const DevState& d, hipStream_t st) {
   if (!d.n) return;
   const int block = 128;
   if (d.n_rep > 1)
-    hipLaunchKernelGGL((k_density<1, true, true, false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_skin_build<true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
   else
-    hipLaunchKernelGGL((k_density<1, false, true, false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_skin_build<false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
 }
 
-// Density pass of skin mode: the same decisions and sums as k_density, over
+// Density pass of skin mode: the same decisions and sums as k_density_q, over
 // the skin list instead of the 27 cells (positions relative to the cells of
 // the last build, with the list's stencil codes), writing the same pair
 // list for the force pass.
