@@ -67,6 +67,12 @@ __device__ __forceinline__ bool decide(float r2, float cut2f, double cut2, const
   return fp64_r2(d, R, s, t) < cut2;
 }
 
+// element t of a slot-indexed array through a 32-bit byte offset (slots are
+// below 2^26): the load takes the scalar-base + vector-offset form
+__device__ __forceinline__ float4 ld_slot(const float4* __restrict__ base, uint32_t t) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (t << 4));
+}
+
 __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s) {
   return reinterpret_cast<const uint4*>(d.nlist) + (size_t)(s >> 6) * ((uint32_t)d.W >> 2) * 64u +
          (s & 63u);
@@ -167,10 +173,10 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
 // sums and the pair list are those of a plain per-candidate loop.
 constexpr uint32_t kDensQ = 8;
 
-template <int ND, bool MULTI, bool RDLE1>
-__global__ void __launch_bounds__(256) k_density_q(DevState d) {
-  extern __shared__ uint2 dens_queue[];  // [kDensQ][blockDim.x]
-  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+template <int ND, bool MULTI, bool RDLE1, int BS>
+__global__ void __launch_bounds__(BS) k_density_q(DevState d) {
+  __shared__ uint2 dens_queue[kDensQ * BS];  // [kDensQ][BS]
+  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (s >= d.n) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
@@ -186,16 +192,21 @@ __global__ void __launch_bounds__(256) k_density_q(DevState d) {
   const int cz = (int)(c / (uint32_t)(nx * ny));
   const uint32_t W = (uint32_t)d.W;
   uint4* lst = const_cast<uint4*>(tile_list(d, s));
-  const uint32_t qs = blockDim.x, qi = threadIdx.x;
-  uint32_t nq = 0;
+  // the lane's queue entries sit BS entries apart from byte offset qi;
+  // [qi, qe) is in use, and past qhi fewer than two places are free
+  constexpr uint32_t qs = BS * (uint32_t)sizeof(uint2);
+  const uint32_t qi = threadIdx.x * (uint32_t)sizeof(uint2);
+  const uint32_t qhi = qi + (kDensQ - 2u) * qs;
+  uint32_t qe = qi;
+  char* const qbase = reinterpret_cast<char*>(dens_queue);
   float rho[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) rho[k] = 0.f;
   uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
   // the queued candidates, in order: decisions, weights, pair-list entries
   auto drain = [&]() {
-    for (uint32_t k = 0; k < nq; ++k) {
-      const uint2 en = dens_queue[qi + k * qs];
+    for (uint32_t k = qi; k < qe; k += qs) {
+      const uint2 en = *reinterpret_cast<const uint2*>(qbase + k);
       const uint32_t e = en.x, t = e & kIndexMask;
       float r2 = __uint_as_float(en.y);
       // outside the guard bands the FP32 r^2 decides; the clamp of w at 0
@@ -231,7 +242,7 @@ __global__ void __launch_bounds__(256) k_density_q(DevState d) {
         ++cnt;
       }
     }
-    nq = 0;
+    qe = qi;
   };
   const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
   const bool wrap = !(cx >= 1 && cx <= nx - 2);
@@ -268,15 +279,15 @@ __global__ void __launch_bounds__(256) k_density_q(DevState d) {
         // The queue keeps room for both (drained when fewer than two are free).
         for (uint32_t t = ta; t < tb; t += 2u) {
           const uint32_t u = min(t + 1u, tb - 1u);
-          const float4 pa = d.spos[t], pb = d.spos[u];
+          const float4 pa = ld_slot(d.spos, t), pb = ld_slot(d.spos, u);
           {
             const bool g2 = t >= b2, g1 = t >= b1;
             const float bx = g2 ? X2 : (g1 ? X1 : X0);
             const float dx = bx - pa.x, dy = by - pa.y, dz = bz - pa.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < cut_hi) {
-              dens_queue[qi + nq * qs] = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
-              ++nq;
+              *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+              qe += qs;
             }
           }
           {
@@ -285,11 +296,11 @@ __global__ void __launch_bounds__(256) k_density_q(DevState d) {
             const float dx = bx - pb.x, dy = by - pb.y, dz = bz - pb.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < cut_hi && u != t) {
-              dens_queue[qi + nq * qs] = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
-              ++nq;
+              *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+              qe += qs;
             }
           }
-          if (__ballot(nq + 2u > kDensQ) != 0ull) drain();
+          if (__ballot(qe > qhi) != 0ull) drain();
         }
       }
     }
@@ -311,13 +322,17 @@ __global__ void __launch_bounds__(256) k_density_q(DevState d) {
 
 template <int ND, bool MULTI>
 static void dens_launch(const DevState& d, int block, hipStream_t st) {
-  const size_t lds = (size_t)kDensQ * (size_t)block * sizeof(uint2);
-  if (d.rd_le_1)
-    hipLaunchKernelGGL((k_density_q<ND, MULTI, true>), dim3(div_up(d.n, block)), dim3(block), lds,
-                       st, d);
-  else
-    hipLaunchKernelGGL((k_density_q<ND, MULTI, false>), dim3(div_up(d.n, block)), dim3(block), lds,
-                       st, d);
+  // the queue lives in a static LDS array sized by the block
+  const dim3 g(div_up(d.n, block));
+  if (d.rd_le_1) {
+    if (block == 64) hipLaunchKernelGGL((k_density_q<ND, MULTI, true, 64>), g, dim3(64), 0, st, d);
+    else if (block == 128) hipLaunchKernelGGL((k_density_q<ND, MULTI, true, 128>), g, dim3(128), 0, st, d);
+    else hipLaunchKernelGGL((k_density_q<ND, MULTI, true, 256>), g, dim3(256), 0, st, d);
+  } else {
+    if (block == 64) hipLaunchKernelGGL((k_density_q<ND, MULTI, false, 64>), g, dim3(64), 0, st, d);
+    else if (block == 128) hipLaunchKernelGGL((k_density_q<ND, MULTI, false, 128>), g, dim3(128), 0, st, d);
+    else hipLaunchKernelGGL((k_density_q<ND, MULTI, false, 256>), g, dim3(256), 0, st, d);
+  }
 }
 
 void launch_skin_build(const DevState& d, hipStream_t st) {

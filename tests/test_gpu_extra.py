@@ -96,13 +96,15 @@ def test_autotuned_launches_are_bit_identical(engine_mod):
     p.cf_s, p.cf_b = 50.0, 10.0
     s = configs.random_chains(600, box, 3)
     res = []
-    for mode in ("default", "manual", "autotune"):
+    for mode in ("default", "manual", "autotune", "manual256"):
         e = engine_mod.Engine(p, capacity=len(s["pos"]))
         e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
         e.set_topology(s["topo"])
         e.set_clock(5, p.pair_seed)
         if mode == "manual":
             e.set_tuning(block_scan=64, block_force=128, nlist_width=128)
+        elif mode == "manual256":
+            e.set_tuning(block_scan=256, block_force=256)
         elif mode == "autotune":
             t = e.autotune(3)
             assert t.block_scan in (64, 128, 256) and t.block_force in (64, 128, 256)
@@ -113,6 +115,7 @@ def test_autotuned_launches_are_bit_identical(engine_mod):
     for k in ("pos", "vel", "acc", "dens"):
         assert np.array_equal(res[0][k], res[1][k]), k
         assert np.array_equal(res[0][k], res[2][k]), k
+        assert np.array_equal(res[0][k], res[3][k]), k
 
 
 @pytest.mark.parametrize("box", [(10.0, 10.0, 8.0), (3.5, 6.0, 6.0)])
