@@ -357,12 +357,91 @@ void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st
   hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
                      full ? 1 : 0, 0, 0, 0);
 }
+
+// The step's permutation (k_permute with the clock advance, the drift and
+// the predicted velocity): every input of the bead is read before anything
+// is written, so a wave has all of its old-set gathers in flight at once
+// (stores into the new set could otherwise alias later loads for the
+// compiler, which then serialises the axes), and with one system the
+// replica constants are wave-uniform.  Same arithmetic as k_permute.
+template <bool ST, bool MULTI>
+__global__ void __launch_bounds__(256) k_step_permute(DevState d) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= d.n) return;
+  const uint32_t i = d.perm[s];
+  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
+  const Replica& R = d.reps[r];
+  const uint32_t cap = d.cap;
+  // the old set is read here for the last time in this step
+  const uint32_t key = d.key[i];
+  double x[3], dl[3], vo[3];
+  float ac[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    x[a] = ld_hint<ST>(&d.pos[a * cap + i]);
+    dl[a] = ld_hint<ST>(&d.delta[a * cap + i]);
+    vo[a] = ld_hint<ST>(&d.vel[a * cap + i]);
+    ac[a] = ld_hint<ST>(&d.acc[a * cap + i]);
+  }
+  const uint32_t id = ld_hint<ST>(&d.id[i]), pr = d.prop[i], am = d.amp_id[i], un = d.unit[i];
+  const uint32_t og = ld_hint<ST>(&d.orig[i]), ch = ld_hint<ST>(&d.chain[i]);
+  // every value in registers before the first store: the compiler would
+  // otherwise sink the loads to their uses, behind the earlier axes' stores
+  asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(dl[0]), "v"(dl[1]), "v"(dl[2]),
+               "v"(vo[0]), "v"(vo[1]), "v"(vo[2]), "v"(ac[0]), "v"(ac[1]), "v"(ac[2]));
+  asm volatile("" ::"v"(id), "v"(pr), "v"(am), "v"(un), "v"(og), "v"(ch), "v"(key));
+  // the replica constants too, before any store
+  const double dt = R.dt;
+  double ib[3], bx[3], h[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ib[a] = R.inv_box[a];
+    bx[a] = R.box[a];
+    h[a] = R.h[a];
+  }
+  const uint32_t c = key - R.cell_off;
+  const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
+  const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
+  float l[3], v[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const DriftAxis o = drift_axis_v(dt, ib[a], bx[a], x[a], vo[a], (double)ac[a], dl[a]);
+    x[a] = o.p;
+    // positions and delta_sumr are next read by the following step's
+    // permutation: streaming stores keep them from displacing this step's
+    // pair-pass data in L2 (vel_buf is read by this step's kick)
+    st_hint<ST>(&d.alt.pos[a * cap + s], o.p);
+    d.alt.vbuf[a * cap + s] = o.vb;
+    st_hint<ST>(&d.alt.delta[a * cap + s], o.dl);
+    l[a] = (float)(o.p - (double)cc[a] * h[a]);
+    v[a] = (float)o.vb;
+  }
+  st_hint<ST>(&d.alt.id[s], id);
+  d.alt.prop[s] = pr;
+  d.alt.amp_id[s] = am;
+  d.alt.unit[s] = un;
+  st_hint<ST>(&d.alt.orig[s], og);
+  st_hint<ST>(&d.alt.chain[s], ch);
+  if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(x[0], x[1], x[2], 0.0);
+  const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
+  d.spos[s] = ps;
+  d.sb[s].pos = ps;
+  d.sb[s].vel = make_float4(v[0], v[1], v[2], 0.f);
+  d.scell[s] = key;
+}
+
 void launch_step_permute(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  if (d.n >= kStreamMin)
-    hipLaunchKernelGGL(k_permute<true>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
-  else
-    hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 1, 1);
+  const bool big = d.n >= kStreamMin, multi = d.n_rep > 1;
+  const dim3 g = grid1(d.n, 256), b(256);
+  if (big) {
+    if (multi) hipLaunchKernelGGL((k_step_permute<true, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_step_permute<true, false>), g, b, 0, st, d);
+  } else {
+    if (multi) hipLaunchKernelGGL((k_step_permute<false, true>), g, b, 0, st, d);
+    else hipLaunchKernelGGL((k_step_permute<false, false>), g, b, 0, st, d);
+  }
 }
 
 // chain bead slots follow the beads to their new slots

@@ -49,17 +49,21 @@ __device__ __forceinline__ uint32_t agg_inc(uint32_t* base, uint32_t key, bool a
 struct DriftAxis {
   double p, dl, vb;
 };
-__device__ __forceinline__ DriftAxis drift_axis(const Replica& R, int a, double x, double v,
-                                                double ac, double dl) {
+// the same with the replica's dt, 1 / box and box edge of the axis as values
+__device__ __forceinline__ DriftAxis drift_axis_v(double dt, double inv_box, double box, double x,
+                                                  double v, double ac, double dl) {
 #pragma clang fp contract(off)
-  const double dt = R.dt;
   const double dr = dt * (v + 0.5 * dt * ac);
   const double tr = x + dr;
   DriftAxis o;
   o.dl = dl + dr;
-  o.p = tr - floor(tr * R.inv_box[a]) * R.box[a];
+  o.p = tr - floor(tr * inv_box) * box;
   o.vb = v + 0.5 * dt * ac;
   return o;
+}
+__device__ __forceinline__ DriftAxis drift_axis(const Replica& R, int a, double x, double v,
+                                                double ac, double dl) {
+  return drift_axis_v(R.dt, R.inv_box[a], R.box[a], x, v, ac, dl);
 }
 
 }  // namespace mdpd
