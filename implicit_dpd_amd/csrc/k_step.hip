@@ -280,17 +280,10 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
 // other master buffer set, and writes the FP32 copies the pair passes read:
 // position inside the bead's cell (FP64 difference, then rounded) with
 // id << 2 | type in .w, and the velocity the force uses.
-// drift: the master set holds the state before this step's drift; the bead
-// is drifted here (same arithmetic as k_drift) and written once, drifted,
-// to its new slot.
-// tick: the step's clock advances here (no kernel of the step before the
-// force pass reads it)
-// ST: streaming loads of the old set and stores of the new set's position,
-// delta_sumr and ids (read again only by the next step)
-template <bool ST>
-__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin,
-                                                 int drift, int tick) {
-  if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
+// This form serves the explicit rebuild (full: also vel, acc and press, and
+// the inverse map for the topology) and skin mode's rebuild steps; the
+// every-step path is k_step_permute, which also drifts the bead.
+__global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin) {
   if (skin && *d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.n) return;
@@ -307,26 +300,13 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   double xs[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    // the old set's position, delta_sumr, velocity and acceleration are
-    // read for the last time in this step: streaming loads
-    double x = ld_hint<ST>(&d.pos[a * cap + i]);
-    double vb, dl = ld_hint<ST>(&d.delta[a * cap + i]);
-    if (drift) {
-      const DriftAxis o = drift_axis(R, a, x, ld_hint<ST>(&d.vel[a * cap + i]),
-                                     (double)ld_hint<ST>(&d.acc[a * cap + i]), dl);
-      x = o.p;
-      dl = o.dl;
-      vb = o.vb;
-    } else {
-      vb = d.vbuf[a * cap + i];
-    }
+    const double x = d.pos[a * cap + i];
+    const double dl = d.delta[a * cap + i];
+    const double vb = d.vbuf[a * cap + i];
     xs[a] = x;
-    // positions and delta_sumr are next read by the following step's
-    // drift and permutation: streaming stores keep them from displacing
-    // this step's pair-pass data in L2 (vel_buf is read by this step's kick)
-    st_hint<ST>(&d.alt.pos[a * cap + s], x);
+    d.alt.pos[a * cap + s] = x;
     d.alt.vbuf[a * cap + s] = vb;
-    st_hint<ST>(&d.alt.delta[a * cap + s], dl);
+    d.alt.delta[a * cap + s] = dl;
     // skin mode: the displacement test of the coming steps starts here
     if (skin) d.dref[a * cap + s] = dl;
     l[a] = (float)(x - (double)cc[a] * R.h[a]);
@@ -337,14 +317,14 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     }
     v[a] = (float)(predicted ? vb : d.vel[a * cap + i]);
   }
-  const uint32_t id = ld_hint<ST>(&d.id[i]), pr = d.prop[i];
-  st_hint<ST>(&d.alt.id[s], id);
+  const uint32_t id = d.id[i], pr = d.prop[i];
+  d.alt.id[s] = id;
   d.alt.prop[s] = pr;
   d.alt.amp_id[s] = d.amp_id[i];
   d.alt.unit[s] = d.unit[i];
-  st_hint<ST>(&d.alt.orig[s], ld_hint<ST>(&d.orig[i]));
-  const uint32_t ch = ld_hint<ST>(&d.chain[i]);
-  st_hint<ST>(&d.alt.chain[s], ch);
+  d.alt.orig[s] = d.orig[i];
+  const uint32_t ch = d.chain[i];
+  d.alt.chain[s] = ch;
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(xs[0], xs[1], xs[2], 0.0);
   const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
   d.spos[s] = ps;
@@ -354,8 +334,8 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
 }
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
-                     full ? 1 : 0, 0, 0, 0);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
+                     full ? 1 : 0, 0);
 }
 
 // The step's permutation (k_permute with the clock advance, the drift and
@@ -810,7 +790,7 @@ void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
 }
 void launch_permute_gated(const DevState& d, hipStream_t st) {
   if (!d.n) return;
-  hipLaunchKernelGGL(k_permute<false>, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1, 0, 0);
+  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1);
 }
 
 // The rebuild's permutation wrote the other buffer set; copy it back so the
