@@ -347,9 +347,9 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first) {
   mark(1);
   launch_scan_cells(d, st);
   launch_scatter(d, st);
-  launch_cell_sort(d, st);
   mark(2);
-  // bead storage moves to the new cell order; the slots below are the new ones
+  // bead storage moves to the new cell order (each cell's beads by previous
+  // slot, ranked inside the permutation); the slots below are the new ones
   launch_step_permute(d, st);
   swap_master(c);
   mark(3);

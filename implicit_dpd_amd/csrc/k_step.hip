@@ -366,6 +366,12 @@ __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
   }
   const uint32_t id = ld_hint<ST>(&d.id[i]), pr = d.prop[i], am = d.amp_id[i], un = d.unit[i];
   const uint32_t og = ld_hint<ST>(&d.orig[i]), ch = ld_hint<ST>(&d.chain[i]);
+  // the scatter left each cell's beads in atomic order; the bead's slot is
+  // its cell's start plus its rank by previous slot among the cell's beads
+  // (the order k_cell_sort gives, without that pass)
+  const uint32_t cb = d.cell_start[key], ce = d.cell_start[key + 1];
+  uint32_t t = cb;
+  for (uint32_t k = cb; k < ce; ++k) t += d.perm[k] < i ? 1u : 0u;
   // every value in registers before the first store: the compiler would
   // otherwise sink the loads to their uses, behind the earlier axes' stores
   asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(dl[0]), "v"(dl[1]), "v"(dl[2]),
@@ -391,24 +397,24 @@ __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
     // positions and delta_sumr are next read by the following step's
     // permutation: streaming stores keep them from displacing this step's
     // pair-pass data in L2 (vel_buf is read by this step's kick)
-    st_hint<ST>(&d.alt.pos[a * cap + s], o.p);
-    d.alt.vbuf[a * cap + s] = o.vb;
-    st_hint<ST>(&d.alt.delta[a * cap + s], o.dl);
+    st_hint<ST>(&d.alt.pos[a * cap + t], o.p);
+    d.alt.vbuf[a * cap + t] = o.vb;
+    st_hint<ST>(&d.alt.delta[a * cap + t], o.dl);
     l[a] = (float)(o.p - (double)cc[a] * h[a]);
     v[a] = (float)o.vb;
   }
-  st_hint<ST>(&d.alt.id[s], id);
-  d.alt.prop[s] = pr;
-  d.alt.amp_id[s] = am;
-  d.alt.unit[s] = un;
-  st_hint<ST>(&d.alt.orig[s], og);
-  st_hint<ST>(&d.alt.chain[s], ch);
+  st_hint<ST>(&d.alt.id[t], id);
+  d.alt.prop[t] = pr;
+  d.alt.amp_id[t] = am;
+  d.alt.unit[t] = un;
+  st_hint<ST>(&d.alt.orig[t], og);
+  st_hint<ST>(&d.alt.chain[t], ch);
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(x[0], x[1], x[2], 0.0);
   const float4 ps = make_float4(l[0], l[1], l[2], __uint_as_float((id << 2) | (pr & 3u)));
-  d.spos[s] = ps;
-  d.sb[s].pos = ps;
-  d.sb[s].vel = make_float4(v[0], v[1], v[2], 0.f);
-  d.scell[s] = key;
+  d.spos[t] = ps;
+  d.sb[t].pos = ps;
+  d.sb[t].vel = make_float4(v[0], v[1], v[2], 0.f);
+  d.scell[t] = key;
 }
 
 void launch_step_permute(const DevState& d, hipStream_t st) {
