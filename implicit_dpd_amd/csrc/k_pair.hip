@@ -285,20 +285,18 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
             const float bx = g2 ? X2 : (g1 ? X1 : X0);
             const float dx = bx - pa.x, dy = by - pa.y, dz = bz - pa.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < cut_hi) {
-              *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
-              qe += qs;
-            }
+            // written always (the queue has room for two more), kept only
+            // in range: no exec-mask branch per candidate
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+            qe += r2 < cut_hi ? qs : 0u;
           }
           {
             const bool g2 = u >= b2, g1 = u >= b1;
             const float bx = g2 ? X2 : (g1 ? X1 : X0);
             const float dx = bx - pb.x, dy = by - pb.y, dz = bz - pb.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < cut_hi && u != t) {
-              *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
-              qe += qs;
-            }
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+            qe += (r2 < cut_hi && u != t) ? qs : 0u;
           }
           if (__ballot(qe > qhi) != 0ull) drain();
         }
