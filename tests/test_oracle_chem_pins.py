@@ -1,0 +1,297 @@
+"""Pins of the oracle's RandomChemEvent (chemmanager.hpp:244-281 with the
+LOCAL_CHEM_EVENT target test DetermineTargetIdWithCorePos, :116-164) against
+geometry fixed by construction, brute-force cone counts and binomial
+statistics (no GPU).
+
+- Target test (:116-164): the core head's membrane normal h2t points from
+  the head to its hydrophobic neighbour CM; the head is moved along h2t by
+  the projection of (hydrophilic CM - head); an amphiphile is a candidate
+  when its head lies in the truncated cone rad <= influ_grd * depth +
+  influ_rad, -influ_hei <= depth <= influ_dep around that apex, and is a
+  target when its draw is <= p_thresld (a draw is always in (0, 1), so
+  p_thresld = 1 takes every candidate and 0 none).  Several cores each add
+  their targets, so a head inside two cones is inserted twice.
+- Insertion (:35-67, :166-173, :260-270): for target head h with unit-2
+  bead t, h_hat = (t - h)/|t - h| (minimum image), a unit tangent t_hat
+  perpendicular to h_hat, bead u of the new amphiphile at
+  wrap(h + eps t_hat + u bond_leng h_hat); ids N + 4i + u, unit u, head
+  Hyphil then three Hyphob, amphiphile id n_amp + i, appended to the
+  topology (the MPI path's AppendTopol, bdf_calculator.hpp:442-543).
+
+The draws are keyed Saru streams in place of the reference's PS::MT
+sequence (absent here), so only what does not depend on the draw values is
+pinned exactly; the draws themselves are pinned through Saru's own vectors.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from implicit_dpd_amd import configs
+from implicit_dpd_amd.capi import MdpdChemParams
+
+from test_oracle_model_pins import cm_brute
+
+HYPHIL, HYPHOB, SOLVENT = configs.HYPHIL, configs.HYPHOB, configs.SOLVENT
+TAG_TANG = 0x5BD1E995
+
+
+def _minimg(d, box):
+    return d - box * np.round(d / box)
+
+
+def _params(box, chem_seed=77):
+    from oracle import pyoracle
+    p = pyoracle.calc_inter_coef(30.0, 50.0, 25.0, 0.1, [3.0] * 6, 0.005)
+    for a in range(3):
+        p.box[a] = box[a]
+    p.cf_s, p.cf_b, p.cf_b_rigid, p.bond_leng = 50.0, 10.0, 10.0, 0.5
+    p.pair_seed, p.chem_seed = 5, chem_seed
+    return p
+
+
+def _chem(core_ids, p_thresld, eps=0.2, rad=2.0, grd=0.5, hei=1.0, dep=3.0, max_amp=10 ** 6):
+    ids = np.ascontiguousarray(core_ids, dtype=np.uint32)
+    cp = MdpdChemParams(p_thresld, eps, rad, hei, dep, grd, max_amp, len(ids),
+                        ids.ctypes.data_as(MdpdChemParams._fields_[8][1]))
+    cp._keep = ids
+    return cp
+
+
+# -------------------------------------------------------- constructed case
+SHIFT = np.array([6.5, 0.0, 0.0])   # moves the core next to the x face
+BOX = np.array([16.0, 16.0, 16.0])
+H = np.array([8.0, 8.0, 8.0])       # core head before the shift
+# (x, y, z) heads of the test amphiphiles before the shift, with the apex at
+# (8, 8, 8.3) and h2t = -z: depth = 8.3 - z, rad = in-plane distance from
+# (8, 8); cone rad <= 0.5 depth + 2, -1 <= depth <= 3
+HEADS = [
+    ((10.5, 8.0, 6.8), True),    # depth 1.5, rad 2.5 <= 2.75
+    ((8.0, 10.8, 6.8), False),   # depth 1.5, rad 2.8 > 2.75
+    ((6.8, 8.0, 9.2), True),     # depth -0.9 >= -1, rad 1.2 <= 1.55
+    ((8.0, 8.5, 9.4), False),    # depth -1.1 < -1
+    ((4.6, 8.0, 5.4), True),     # depth 2.9 <= 3, rad 3.4 <= 3.45
+    ((8.0, 8.0, 5.2), False),    # depth 3.1 > 3
+    ((8.0, 5.8, 7.8), True),     # depth 0.5, rad 2.2 <= 2.25
+    ((5.7, 8.0, 7.8), False),    # depth 0.5, rad 2.3 > 2.25
+]
+
+
+def _constructed():
+    """Amphiphile 0 is the core: head at H, tail straight down (-z), so its
+    hydrophobic neighbours within 1.2 are units 1 and 2 (CM (8, 8, 7.25),
+    h2t = -z).  A free Hyphil bead at (8.3, 8, 8.6) makes the hydrophilic CM
+    (8.15, 8, 8.3): the projection moves the apex to (8, 8, 8.3) and drops
+    the in-plane offset.  Test amphiphiles have their tails pointing away
+    from H, so no other bead enters the core head's CM sums."""
+    pos, prop = [], []
+    for u in range(4):
+        pos.append(H + [0, 0, -0.5 * u])
+        prop.append(HYPHIL if u == 0 else HYPHOB)
+    dirs = []
+    for h, _ in HEADS:
+        h = np.array(h)
+        e = (h - H) / np.linalg.norm(h - H)
+        dirs.append(e)
+        # units 1, 2 along e, unit 3 bent off it (h_hat is taken to unit 2)
+        kink = np.cross(e, [0.0, 0.0, 1.0] if abs(e[2]) < 0.9 else [1.0, 0.0, 0.0])
+        kink /= np.linalg.norm(kink)
+        for u, q in enumerate((0.0, 0.5 * e, 1.0 * e, 1.0 * e + 0.5 * kink)):
+            pos.append(h + q)
+            prop.append(HYPHIL if u == 0 else HYPHOB)
+    pos.append(np.array([8.3, 8.0, 8.6]))
+    prop.append(HYPHIL)
+    pos = np.array(pos) + SHIFT
+    pos -= np.floor(pos / BOX) * BOX
+    n = len(pos)
+    n_amp = 1 + len(HEADS)
+    amp_id = np.concatenate([np.repeat(np.arange(n_amp), 4), [0xFFFFFFFF]]).astype(np.uint32)
+    unit = np.concatenate([np.tile(np.arange(4), n_amp), [0xFFFFFFFF]]).astype(np.uint32)
+    return dict(pos=pos, prop=np.array(prop, np.uint32), ids=np.arange(n, dtype=np.uint32),
+                amp_id=amp_id, unit=unit, topo=np.arange(4 * n_amp, dtype=np.uint32),
+                vel=np.zeros((n, 3)), dirs=[np.array([0.0, 0.0, -1.0])] + dirs)
+
+
+def _run_event(oracle, s, cp, time=40, extra=64, core_amp=(0,)):
+    p = _params(BOX if "box" not in s else s["box"])
+    n = len(s["pos"])
+    n_amp = len(s["topo"]) // 4
+    o = oracle.System(p, n + 4 * extra, n_amp + extra)
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    o.set_topology(s["topo"], core_amp_id=list(core_amp))
+    o.set_clock(time)
+    k, stop = o.chem_event(cp)
+    return p, o, k, stop
+
+
+def test_constructed_cone_targets_and_geometry(oracle):
+    s = _constructed()
+    # the CM sums the target test starts from, by brute force
+    hl, hb, _, _ = cm_brute(s["pos"], s["prop"], BOX)
+    assert np.allclose(hb[0], (H + [0, 0, -0.75] + SHIFT) % BOX, atol=1e-12)
+    assert np.allclose(hl[0], (H + [0.15, 0, 0.3] + SHIFT) % BOX, atol=1e-12)
+    n0, amp0 = len(s["pos"]), len(s["topo"]) // 4
+    eps = 0.2
+    p, o, k, stop = _run_event(oracle, s, _chem([0], 1.0, eps=eps))
+    # the core itself (depth 0.3, rad 0) and the four heads marked inside
+    expect = [0] + [1 + i for i, (_, inside) in enumerate(HEADS) if inside]
+    assert k == len(expect) and stop == 0
+    r = o.get()
+    topo = o.topology()
+    assert len(r["pos"]) == n0 + 4 * k
+    for i, a in enumerate(expect):
+        head = s["pos"][4 * a]
+        h_hat = s["dirs"][a]
+        new = slice(n0 + 4 * i, n0 + 4 * i + 4)
+        assert np.array_equal(r["id"][new], n0 + 4 * i + np.arange(4))
+        assert np.array_equal(r["prop"][new], [HYPHIL, HYPHOB, HYPHOB, HYPHOB])
+        assert np.array_equal(r["amp_id"][new], np.full(4, amp0 + i))
+        assert np.array_equal(r["unit"][new], np.arange(4))
+        assert np.array_equal(topo[4 * (amp0 + i):4 * (amp0 + i) + 4], n0 + 4 * i + np.arange(4))
+        d = _minimg(r["pos"][new] - head, BOX)          # offsets from the target head
+        t = d[0] / eps                                  # the unit tangent
+        assert math.isclose(np.linalg.norm(t), 1.0, rel_tol=1e-12)
+        assert abs(t @ h_hat) < 1e-12
+        for u in range(4):
+            assert np.allclose(d[u], eps * t + u * 0.5 * h_hat, atol=1e-12)
+        assert np.all((r["pos"][new] >= 0) & (r["pos"][new] < BOX))
+        assert np.all(np.isfinite(r["vel"][new]))
+        assert np.all(r["delta"][new] == 0) and np.all(r["vel_buf"][new] == 0)
+    # the old beads are untouched
+    for key in ("pos", "vel"):
+        assert np.array_equal(r[key][:n0], s[key])
+
+
+def test_tangent_follows_keyed_draw(oracle):
+    """t_hat = normalise(b - h_hat (b . h_hat)) with b = (cos th, sin th, 0)
+    and th = 2 pi u, u the first uni draw of Saru(new amphiphile id, time,
+    chem_seed ^ TAG_TANG) (GetMembraneTangRandomVec, chemmanager.hpp:166-173,
+    with the keyed stream in place of genrand_real1)."""
+    s = _constructed()
+    n0, amp0 = len(s["pos"]), len(s["topo"]) // 4
+    p, o, k, _ = _run_event(oracle, s, _chem([0], 1.0, eps=0.3), time=1234)
+    r = o.get()
+    expect = [0] + [1 + i for i, (_, inside) in enumerate(HEADS) if inside]
+    seeds = np.arange(amp0, amp0 + k, dtype=np.uint32)
+    _, uni, _ = oracle.saru_batch(seeds, np.full(k, 1234, np.uint32),
+                                  np.full(k, p.chem_seed ^ TAG_TANG, np.uint32))
+    for i, a in enumerate(expect):
+        th = 2 * math.pi * uni[i, 0]
+        b = np.array([math.cos(th), math.sin(th), 0.0])
+        h = s["dirs"][a]
+        t = b - h * (b @ h)
+        t /= np.linalg.norm(t)
+        d = _minimg(r["pos"][n0 + 4 * i] - s["pos"][4 * a], BOX)
+        assert np.allclose(d, 0.3 * t, atol=1e-12)
+
+
+def test_threshold_zero_and_two_cores(oracle):
+    s = _constructed()
+    n0 = len(s["pos"])
+    _, o, k, stop = _run_event(oracle, s, _chem([0], 0.0))
+    assert k == 0 and stop == 0 and len(o.get()["pos"]) == n0
+    # the same core listed twice: every candidate is a target once per core,
+    # in core order, then amphiphile order
+    _, o, k, _ = _run_event(oracle, s, _chem([0, 0], 1.0))
+    expect = [0] + [1 + i for i, (_, inside) in enumerate(HEADS) if inside]
+    assert k == 2 * len(expect)
+    r = o.get()
+    heads = [s["pos"][4 * a] for a in expect + expect]
+    for i, h in enumerate(heads):
+        assert math.isclose(np.linalg.norm(_minimg(r["pos"][n0 + 4 * i] - h, BOX)), 0.2, rel_tol=1e-12)
+
+
+def test_stop_flag_at_max_amp_num(oracle):
+    s = _constructed()
+    amp0 = len(s["topo"]) // 4
+    _, _, k, stop = _run_event(oracle, s, _chem([0], 1.0, max_amp=amp0 + 5))
+    assert k == 5 and stop == 1
+    _, _, k, stop = _run_event(oracle, s, _chem([0], 1.0, max_amp=amp0 + 6))
+    assert k == 5 and stop == 0
+
+
+# -------------------------------------------------------- random membrane
+def _cone_candidates(s, box, core_amp, rad0, grd, hei, dep):
+    """Brute-force candidate amphiphiles of one core (numpy)."""
+    hl, hb, _, _ = cm_brute(s["pos"], s["prop"], box)
+    hi = s["topo"][4 * core_amp]
+    ch, ct, ccm = s["pos"][hi], hb[hi], hl[hi]
+    h2t = _minimg(ct - ch, box)
+    h2t /= np.linalg.norm(h2t)
+    apex = ch + (_minimg(ccm - ch, box) @ h2t) * h2t
+    apex -= np.floor(apex / box) * box
+    heads = s["pos"][s["topo"][0::4]]
+    c2p = _minimg(heads - apex, box)
+    depth = c2p @ h2t
+    rad = np.linalg.norm(c2p - depth[:, None] * h2t, axis=1)
+    return np.nonzero((rad <= grd * depth + rad0) & (depth >= -hei) & (depth <= dep))[0]
+
+
+def _membrane(seed):
+    box = np.array([12.0, 12.0, 10.0])
+    rng = np.random.default_rng(seed)
+    n_amp = 700
+    # a rough bilayer: heads on two sheets, tails pointing inwards
+    xy = rng.random((n_amp, 2)) * box[:2]
+    up = rng.random(n_amp) < 0.5
+    z = np.where(up, 6.6, 3.4) + rng.normal(scale=0.2, size=n_amp)
+    head = np.column_stack([xy, z])
+    d = np.column_stack([rng.normal(scale=0.2, size=(n_amp, 2)), np.where(up, -1.0, 1.0)])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = (head[:, None, :] + 0.5 * np.arange(4)[None, :, None] * d[:, None, :])
+    pos[:, 1:] += rng.normal(scale=0.15, size=(n_amp, 3, 3))  # bent chains
+    pos = pos.reshape(-1, 3)
+    pos -= np.floor(pos / box) * box
+    n = len(pos)
+    return dict(box=box, pos=pos, vel=np.zeros((n, 3)), ids=np.arange(n, dtype=np.uint32),
+                prop=np.tile(np.array([HYPHIL, HYPHOB, HYPHOB, HYPHOB], np.uint32), n_amp),
+                amp_id=np.repeat(np.arange(n_amp, dtype=np.uint32), 4),
+                unit=np.tile(np.arange(4, dtype=np.uint32), n_amp),
+                topo=np.arange(n, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_candidates_equal_brute_force_cone(oracle, seed):
+    s = _membrane(seed)
+    box = s["box"]
+    cone = dict(rad0=2.5, grd=0.3, hei=1.5, dep=2.0)
+    core = [3, 100]
+    cand = [_cone_candidates(s, box, c, **cone) for c in core]
+    assert all(len(c) > 10 for c in cand)
+    core_ids = [int(s["topo"][4 * c]) for c in core]
+    cp = _chem(core_ids, 1.0, eps=0.25, rad=cone["rad0"], grd=cone["grd"], hei=cone["hei"],
+               dep=cone["dep"])
+    n0 = len(s["pos"])
+    p, o, k, _ = _run_event(oracle, s, cp, extra=4 * 400, core_amp=core)
+    expect = np.concatenate(cand)
+    assert k == len(expect)
+    r = o.get()
+    # each new amphiphile's head sits eps from its target's head
+    for i, a in enumerate(expect):
+        d = np.linalg.norm(_minimg(r["pos"][n0 + 4 * i] - s["pos"][4 * a], box))
+        assert math.isclose(d, 0.25, rel_tol=1e-10), (i, a, d)
+        h_hat = _minimg(s["pos"][4 * a + 2] - s["pos"][4 * a], box)
+        h_hat /= np.linalg.norm(h_hat)
+        step = _minimg(r["pos"][n0 + 4 * i + 3] - r["pos"][n0 + 4 * i], box)
+        assert np.allclose(step, 1.5 * h_hat, atol=1e-10)
+    # Box-Muller velocities of the keyed draws: mean 0, variance kT = 1
+    v = r["vel"][n0:].ravel()
+    assert abs(v.mean()) < 5 / math.sqrt(len(v)) and abs(v.var() - 1) < 5 * math.sqrt(2 / len(v))
+
+
+def test_target_count_is_binomial(oracle):
+    s = _membrane(3)
+    box = s["box"]
+    cone = dict(rad0=4.0, grd=0.0, hei=2.0, dep=3.0)
+    cand = _cone_candidates(s, box, 10, **cone)
+    m = len(cand)
+    assert m > 40
+    core_id = int(s["topo"][40])
+    total, trials, pthr = 0, 12, 0.3
+    for t in range(trials):
+        cp = _chem([core_id], pthr, rad=cone["rad0"], grd=0.0, hei=cone["hei"], dep=cone["dep"])
+        _, _, k, _ = _run_event(oracle, s, cp, time=500 + t, extra=m + 4, core_amp=[10])
+        assert 0 <= k <= m
+        total += k
+    mean, sd = trials * m * pthr, math.sqrt(trials * m * pthr * (1 - pthr))
+    assert abs(total - mean) < 5 * sd, (total, mean, sd)
