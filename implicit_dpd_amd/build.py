@@ -28,7 +28,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 SOURCES = ["k_step.hip", "k_pair.hip", "k_obs.hip", "k_mlp.hip", "k_config.hip", "engine.cpp", "params.cpp",
-           "ext.cpp", "mlp.cpp", "config.cpp", "io.cpp"]
+           "ext.cpp", "mlp.cpp", "config.cpp", "io.cpp", "rdf_noalloc.hip"]
 
 
 def _compile(src):

@@ -195,6 +195,7 @@ PROTOTYPES = {
     "mdpd_observe_replicas": (C.c_int, [VP, P(MdpdObservables)]),
     "mdpd_density_profile": (C.c_int, [VP, C.c_int32, C.c_int32, _DP]),
     "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
+    "mdpd_rdf_noalloc": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
     "mdpd_write_xyz_async": (C.c_int, [VP, C.c_char_p, C.c_uint32, C.c_int32]),
     "mdpd_io_wait": (C.c_int, [VP]),
     "mdpd_read_xyz_header": (C.c_int, [C.c_char_p, _U32P, _U32P]),

@@ -63,6 +63,9 @@ void invalidate_graphs(mdpd_ctx* c);
 void swap_master(mdpd_ctx* c);
 int run_steps(mdpd_ctx* c, uint32_t nsteps);
 int check_flags(mdpd_ctx* c);
+// device address of the fixed scratch of mdpd_rdf_noalloc on device `dev`
+// (looked up once per device, at context creation)
+int noalloc_scratch_init(int dev);
 }  // namespace mdpd
 
 #define HIPCHK(c, call)                                                                    \

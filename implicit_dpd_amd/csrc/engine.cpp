@@ -155,6 +155,11 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   }
   mdpd_ctx* c = new mdpd_ctx();
   hipGetDevice(&c->dev);
+  if (int rc = noalloc_scratch_init(c->dev)) {
+    const std::string why2 = g_err;
+    delete c;
+    return fail(nullptr, rc, why2);
+  }
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(nullptr, MDPD_ERR_HIP, "hipStreamCreate failed");

@@ -254,6 +254,15 @@ int mdpd_checkpoint_load(mdpd_ctx* ctx, const char* path);
    needs the cell list of the current positions (after mdpd_run or
    mdpd_rebuild_cells). */
 int mdpd_rdf(mdpd_ctx* ctx, double rmax, int32_t nbins, double* out);
+/* mdpd_rdf without any device allocation: no counterpart in the reference.
+   For callers that cannot allocate device memory (code under graph capture,
+   tight serving loops).  Its scratch (bin edges, counters) is one fixed
+   device array of 1,057,024 bytes per GPU, held until the process ends, so
+   n_rep * nbins may be at most 65536 (more gives MDPD_ERR_CAPACITY, before
+   any work).  Same result as mdpd_rdf, bit for bit.  All calls on one GPU
+   share that array: calls that overlap on a GPU race, and the caller must
+   serialise them. */
+int mdpd_rdf_noalloc(mdpd_ctx* ctx, double rmax, int32_t nbins, double* out);
 /* Chain geometry histograms, the quantities of
    analysis/bond_angle/h2e.cpp:64-104, out[n_rep][4][nbins]: row 0 the first
    bond |r1 - r0|, row 1 the other bonds (bins of bond_max / nbins, on the
