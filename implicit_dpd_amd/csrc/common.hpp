@@ -15,6 +15,7 @@ enum : uint32_t {
   FLAG_NLIST_OVERFLOW = 1u,
   FLAG_NONFINITE = 2u,
   FLAG_OUT_OF_BOX = 4u,
+  FLAG_CHEM_CAPACITY = 8u,  // a chemistry event would have exceeded the capacity
 };
 
 // Per-replica constants, device resident.  FP32 copies of the pair

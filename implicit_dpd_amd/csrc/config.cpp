@@ -415,6 +415,7 @@ int mdpd_make_config(mdpd_ctx* c, const mdpd_config_spec* s, double* kin_temp) {
     d.n = 0;
     d.n_amp = 0;
     c->n_amp = 0;
+    set_launch_bound(c);
     c->rep_off[1] = 0;
     c->R[0].n_beads = 0;
     c->have_cells = c->cells_fresh = c->have_density = false;
@@ -438,6 +439,7 @@ int mdpd_make_config(mdpd_ctx* c, const mdpd_config_spec* s, double* kin_temp) {
   d.n = n;
   d.n_amp = na;
   c->n_amp = na;
+  set_launch_bound(c);
   c->rep_off[1] = n;
   c->R[0].n_beads = n;
   HIPCHK(c, hipMemcpyAsync(d.rep_off, c->rep_off.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, c->st));

@@ -43,8 +43,22 @@ struct mdpd_ctx {
   // the bead storage (cleared by every call that moves or reorders beads)
   bool skin_valid = false;
   // captured steps by storage parity and variant (1: virial step, 2: CM-drift
-  // step, 4: the step computes its own cell keys)
-  hipGraphExec_t gexec[2][8] = {};
+  // step, 4: the step computes its own cell keys, 8: the step ends with the
+  // chemistry event)
+  hipGraphExec_t gexec[2][16] = {};
+  // chemistry (single system).  Slot 0 of chem_dev / chem_state serves the
+  // event inside mdpd_run, slot 1 the explicit mdpd_chem_event; the counts
+  // and targets scratch is shared (the two never overlap on the stream).
+  mdpd::ChemDev* chem_dev = nullptr;      // [2]
+  mdpd::ChemState* chem_state = nullptr;  // [2]
+  uint32_t* chem_counts = nullptr;        // [kMaxCore * chem_count_blocks(cap)]
+  uint32_t* chem_targets = nullptr;       // [amp_cap]
+  uint32_t* run_ok_word = nullptr;        // the device's run word of grow mode
+  mdpd::ChemDev chem_host[2] = {};        // host copies (sources of the uploads)
+  bool grow = false;          // chemistry may append beads: see DevState::nl
+  bool chem_on = false;       // mdpd_run ends steps with time >= beg_chem with an event
+  bool chem_stopped = false;  // an in-run event reached max_amp_num
+  uint32_t beg_chem = 0;
   int parity = 0;            // which buffer set is current (flips every rebuild)
   uint32_t* inv_orig = nullptr;  // scratch [cap]
   bool profiling = false;
@@ -66,6 +80,16 @@ int check_flags(mdpd_ctx* c);
 // device address of the fixed scratch of mdpd_rdf_noalloc on device `dev`
 // (looked up once per device, at context creation)
 int noalloc_scratch_init(int dev);
+// grow mode: per-bead launches cover the capacity and kernels read the live
+// counts on the device, so chemistry never forces a graph re-capture
+int enter_grow(mdpd_ctx* c);
+inline void set_launch_bound(mdpd_ctx* c) { c->d.nl = (c->grow && c->d.n) ? c->cap : c->d.n; }
+// the host's copies of the counts after device-side chemistry
+int pull_counts(mdpd_ctx* c);
+uint32_t chem_count_blocks(uint32_t cap);
+void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
+                       uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
+                       hipStream_t st);
 }  // namespace mdpd
 
 #define HIPCHK(c, call)                                                                    \

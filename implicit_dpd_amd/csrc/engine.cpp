@@ -34,6 +34,35 @@ void invalidate_graphs(mdpd_ctx* c) {
       }
 }
 
+int enter_grow(mdpd_ctx* c) {
+  if (c->n_rep != 1 || !c->chem_dev)
+    return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
+  if (!c->grow) {
+    c->grow = true;
+    c->d.run_ok = c->run_ok_word;
+    set_launch_bound(c);
+    invalidate_graphs(c);
+  }
+  return MDPD_OK;
+}
+
+int pull_counts(mdpd_ctx* c) {
+  uint32_t v[3];
+  HIPCHK(c, hipMemcpyAsync(&v[0], c->d.rep_off + 1, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(&v[1], c->d.amp_off + 1, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(&v[2], c->d_amp_num, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->d.n = v[0];
+  c->rep_off[1] = v[0];
+  c->R[0].n_beads = v[0];
+  c->d.n_amp = v[1];
+  c->n_amp = v[1];
+  c->amp_off[1] = v[1];
+  c->amp_num_rep[0] = v[2];
+  set_launch_bound(c);
+  return MDPD_OK;
+}
+
 // the permutation wrote the other buffer set: make it the current one
 void swap_master(mdpd_ctx* c) {
   Master cur = static_cast<Master&>(c->d);
@@ -138,6 +167,22 @@ Replica make_replica(const mdpd_params& p, uint32_t cell_off, uint32_t bead_off,
       for (int k = 0; k < 3; ++k) r.mb[i][j][k] = (float)p.mb[i][j][k];
     }
   return r;
+}
+
+// the chemistry slots' state: no core head found yet, no events; the run word set
+int chem_init_slots(mdpd_ctx* c) {
+  static ChemState fresh;
+  static bool once = false;
+  if (!once) {
+    memset(&fresh, 0, sizeof(fresh));
+    for (uint32_t k = 0; k < kMaxCore; ++k) fresh.core_slot[k] = 0xffffffffu;
+    once = true;
+  }
+  for (int q = 0; q < 2; ++q)
+    HIPCHK(c, hipMemcpyAsync(c->chem_state + q, &fresh, sizeof(ChemState), hipMemcpyHostToDevice, c->st));
+  launch_fill_u32(c->run_ok_word, 1, 1u, c->st);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDPD_OK;
 }
 
 int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t capacity,
@@ -268,9 +313,18 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(c->vir_bonded, (size_t)n_rep * 3);
   ALLOC(c->obs_out, (size_t)n_rep * 16);
   ALLOC(c->d_amp_num, n_rep);
+  if (n_rep == 1) {
+    ALLOC(c->chem_dev, 2);
+    ALLOC(c->chem_state, 2);
+    ALLOC(c->chem_counts, (size_t)kMaxCore * chem_count_blocks(cap));
+    ALLOC(c->chem_targets, c->amp_cap);
+    ALLOC(c->run_ok_word, 1);
+  }
 #undef ALLOC
   d.cap = cap;
   d.n = 0;
+  d.nl = 0;
+  d.run_ok = nullptr;
   d.ncell = cells;
   d.W = c->tune.nlist_width;
   d.n_rep = n_rep;
@@ -285,6 +339,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
                      hipMemcpyHostToDevice, c->st) != hipSuccess ||
       hipMemcpyAsync(d.rep_cell_off, c->cell_off.data(), sizeof(uint32_t) * (n_rep + 1),
                      hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      (n_rep == 1 && chem_init_slots(c) != MDPD_OK) ||
       hipStreamSynchronize(c->st) != hipSuccess) {
     mdpd_destroy(c);
     return fail(nullptr, MDPD_ERR_HIP, "initial upload failed");
@@ -297,7 +352,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
 // of the drifted positions come from a key pass (the first step of a run,
 // and the step after a CM-drift removal, which changed the velocities);
 // otherwise the previous step's force epilogue wrote them.
-void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first) {
+void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first, bool chem) {
   const DevState& d = c->d;
   hipStream_t st = c->st;
   const bool prof = c->profiling;
@@ -375,6 +430,11 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first) {
   // (main.cpp order: after it) may run inside the force pass
   if (cm) launch_cm_drift(d, st);
   mark(8);
+  // RandomChemEvent (main.cpp:186-196); new beads join the next step (their
+  // cell keys with the others' when this step's force pass binned them)
+  if (chem)
+    launch_chem_event(d, c->chem_dev, c->chem_state, c->chem_host[0].n_core, c->chem_counts,
+                      c->chem_targets, c->d_amp_num, !virial && !cm, st);
   mark(9);
   mark(10);
 }
@@ -392,6 +452,8 @@ int mdpd::check_flags(mdpd_ctx* c) {
       return fail(c, MDPD_ERR_NLIST_OVERFLOW,
                   "a bead has more neighbours than nlist_width = " + std::to_string(c->d.W));
     if (f & FLAG_NONFINITE) return fail(c, MDPD_ERR_NONFINITE, "non-finite position/velocity/force");
+    if (f & FLAG_CHEM_CAPACITY)
+      return fail(c, MDPD_ERR_CAPACITY, "inserted amphiphiles would exceed the capacity");
     return fail(c, MDPD_ERR_OUT_OF_BOX, "bead outside the box");
   }
   return MDPD_OK;
@@ -399,6 +461,10 @@ int mdpd::check_flags(mdpd_ctx* c) {
 
 int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   const bool skin = c->tune.verlet_skin > 0.0f;
+  if (c->chem_on && skin)
+    return fail(c, MDPD_ERR_UNSUPPORTED,
+                "chemistry inside mdpd_run needs the every-step cell list (verlet_skin = 0)");
+  const uint32_t time0 = c->time;
   // skin mode: the first step builds when the host has moved or reordered
   // beads since the last build
   if (skin && nsteps && !c->skin_valid) launch_fill_u32(c->d.rebuild, 1, 1u, c->st);
@@ -410,8 +476,9 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
     // the last step of a run is its virial step
     const bool virial = (k + 1 == nsteps);
     const bool cm = (c->time % c->cm_corr) == 0;
+    const bool chem = c->chem_on && c->time >= c->beg_chem;
     if (c->tune.use_graph && !c->profiling) {
-      const int v = (virial ? 1 : 0) | (cm ? 2 : 0) | (first && !skin ? 4 : 0);
+      const int v = (virial ? 1 : 0) | (cm ? 2 : 0) | (first && !skin ? 4 : 0) | (chem ? 8 : 0);
       // (a skin-mode step keeps the storage where it is: no parity flip)
       const int p = c->parity;
       if (!c->gexec[p][v]) {
@@ -419,7 +486,7 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
         // running it does
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, virial, cm, first);
+        enqueue_step(c, virial, cm, first, chem);
         HIPCHK(c, hipStreamEndCapture(c->st, &g));
         HIPCHK(c, hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0));
         hipGraphDestroy(g);
@@ -428,10 +495,27 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
       }
       HIPCHK(c, hipGraphLaunch(c->gexec[p][v], c->st));
     } else {
-      enqueue_step(c, virial, cm, first);
+      enqueue_step(c, virial, cm, first, chem);
     }
     first = cm;
     c->time++;
+  }
+  if (c->chem_on && nsteps) {
+    // the chemistry changed the counts on the device; a stop skipped the
+    // run's remaining steps
+    uint32_t w[2] = {1u, 0u};
+    HIPCHK(c, hipMemcpyAsync(&w[0], c->run_ok_word, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&w[1], c->d.d_time, 4, hipMemcpyDeviceToHost, c->st));
+    int rc = pull_counts(c);
+    if (rc) return rc;
+    if (w[0] == 0u) {
+      const uint32_t done = w[1] - (time0 - 1u);  // steps whose permutation ran
+      if ((nsteps - done) & 1u) swap_master(c);  // undo the skipped steps' parity flips
+      c->time = w[1];  // main.cpp:188 leaves the loop before time++
+      c->chem_stopped = true;
+      launch_fill_u32(c->run_ok_word, 1, 1u, c->st);
+      HIPCHK(c, hipStreamSynchronize(c->st));
+    }
   }
   c->drifted = false;
   c->have_cells = true;
@@ -565,10 +649,17 @@ int mdpd_upload_soa(mdpd_ctx* c, uint32_t n, const double* pos, const double* ve
   }
   if (n != d.n) invalidate_graphs(c);
   d.n = n;
+  set_launch_bound(c);
   if (c->n_rep == 1) {
     c->rep_off[1] = n;
     c->R[0].n_beads = n;
+    // no chains until set_topology (the device's copies are the live counts
+    // in grow mode)
+    c->amp_off.assign(2, 0);
+    c->amp_num_rep[0] = 0;
     HIPCHK(c, hipMemcpyAsync(d.rep_off, c->rep_off.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(d.amp_off, c->amp_off.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_amp_num, c->amp_num_rep.data(), sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
   c->drifted = false;
@@ -963,6 +1054,9 @@ int mdpd_head_conversion(mdpd_ctx* c) {
 int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
   if (!c) return MDPD_ERR_ARG;
   if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->chem_on && c->chem_stopped)
+    return fail(c, MDPD_ERR_STATE, "the chemistry stopped the run (max_amp_num reached); "
+                                   "mdpd_set_chemistry starts it again");
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
   int rc = run_steps(c, nsteps);
   if (rc) return rc;
@@ -972,6 +1066,9 @@ int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
 int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
   if (!c || !gpu_ms) return MDPD_ERR_ARG;
   if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->chem_on && c->chem_stopped)
+    return fail(c, MDPD_ERR_STATE, "the chemistry stopped the run (max_amp_num reached); "
+                                   "mdpd_set_chemistry starts it again");
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
@@ -989,6 +1086,56 @@ int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
   hipEventDestroy(e1);
   if (rc) return rc;
   return check_flags(c);
+}
+
+int mdpd_set_chemistry(mdpd_ctx* c, const mdpd_chem_params* cp, uint32_t beg_chem, int32_t enable) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!enable) {
+    c->chem_on = false;
+    c->chem_stopped = false;
+    return MDPD_OK;
+  }
+  if (!cp) return fail(c, MDPD_ERR_ARG, "null chemistry parameters");
+  if (cp->n_core > kMaxCore) return fail(c, MDPD_ERR_ARG, "at most 256 core amphiphiles");
+  if (cp->n_core && !cp->core_ptcl_id) return fail(c, MDPD_ERR_ARG, "core_ptcl_id is null");
+  if (c->tune.verlet_skin > 0.0f)
+    return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry inside mdpd_run needs verlet_skin = 0");
+  int rc = enter_grow(c);
+  if (rc) return rc;
+  ChemDev& h = c->chem_host[0];
+  if (h.n_core != cp->n_core) invalidate_graphs(c);  // the launch grids depend on it
+  h = ChemDev{};
+  h.p_thresld = cp->p_thresld;
+  h.eps = cp->eps;
+  h.influ_rad = cp->influ_rad;
+  h.influ_hei = cp->influ_hei;
+  h.influ_dep = cp->influ_dep;
+  h.influ_grd = cp->influ_grd;
+  h.bond_leng = c->P[0].bond_leng;
+  h.cf_b = c->P[0].cf_b;
+  h.max_amp_num = cp->max_amp_num;
+  h.n_core = cp->n_core;
+  h.in_run = 1;
+  for (uint32_t k = 0; k < cp->n_core; ++k) h.core_id[k] = cp->core_ptcl_id[k];
+  HIPCHK(c, hipMemcpyAsync(c->chem_dev, &h, sizeof(ChemDev), hipMemcpyHostToDevice, c->st));
+  if ((rc = chem_init_slots(c))) return rc;
+  c->chem_on = true;
+  c->chem_stopped = false;
+  c->beg_chem = beg_chem;
+  return MDPD_OK;
+}
+
+int mdpd_chem_status(mdpd_ctx* c, int32_t* stopped, uint32_t* events, uint32_t* inserted) {
+  if (!c || !stopped) return MDPD_ERR_ARG;
+  *stopped = c->chem_stopped ? 1 : 0;
+  uint32_t v[8] = {0};
+  if (c->chem_state) {
+    HIPCHK(c, hipMemcpyAsync(v, c->chem_state, sizeof(v), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  if (events) *events = v[5];
+  if (inserted) *inserted = v[6];
+  return MDPD_OK;
 }
 
 int mdpd_skin_stats(mdpd_ctx* c, uint64_t* builds, uint64_t* entries) {

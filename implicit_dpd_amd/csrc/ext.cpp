@@ -15,133 +15,53 @@
 
 using namespace mdpd;
 
-namespace {
-
-void min_image_round(const double* box, double d[3]) {
-  for (int a = 0; a < 3; ++a) d[a] -= box[a] * std::round(d[a] * (1.0 / box[a]));
-}
-void normalize(double v[3]) {
-  const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  for (int a = 0; a < 3; ++a) v[a] /= l;
-}
-
-}  // namespace
-
 extern "C" {
 
 // ChemManager::RandomChemEvent, serial (chemmanager.hpp:244-281) with
-// DetermineTargetIdWithCorePos (:116-164) and GetCorePos (:84-114); the
-// cone test and insertion run on the device, the per-core geometry (a few
-// vectors) on the host.
+// DetermineTargetIdWithCorePos (:116-164) and GetCorePos (:84-114): the
+// whole event runs on the device (the kernels mdpd_run uses inside its
+// steps), with the context's preallocated scratch; the host reads back the
+// counts once.  `time` is the context clock.
 int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
   if (c) c->skin_valid = false;
   if (!c || !cp || !stop) return fail(c, MDPD_ERR_ARG, "null argument");
   if (c->n_rep != 1) return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
   if (!c->cells_fresh) return fail(c, MDPD_ERR_STATE, "chemistry needs the cell list of the current positions");
   if (cp->n_core && !cp->core_ptcl_id) return fail(c, MDPD_ERR_ARG, "core_ptcl_id is null");
+  if (cp->n_core > kMaxCore) return fail(c, MDPD_ERR_ARG, "at most 256 core amphiphiles");
+  int rc = enter_grow(c);
+  if (rc) return rc;
   DevState& d = c->d;
-  const uint32_t nc = cp->n_core;
-  const double* box = c->R[0].box;
-  std::vector<uint32_t> targets;
-  // the step keeps chain[] current, not topo: rebuild it for the slots now
-  launch_topo_from_chain(d, c->st);
-  if (nc && d.n_amp) {
-    uint32_t *ids = nullptr, *idx = nullptr;
-    double *cm = nullptr, *core = nullptr;
-    uint8_t* flags = nullptr;
-    HIPCHK(c, hipMalloc(&ids, 4 * nc));
-    HIPCHK(c, hipMalloc(&idx, 4 * nc));
-    HIPCHK(c, hipMalloc(&cm, 64 * nc));
-    HIPCHK(c, hipMalloc(&core, 48 * nc));
-    HIPCHK(c, hipMalloc(&flags, (size_t)nc * d.n_amp));
-    HIPCHK(c, hipMemcpyAsync(ids, cp->core_ptcl_id, 4 * nc, hipMemcpyHostToDevice, c->st));
-    launch_fill_u32(idx, nc, 0xffffffffu, c->st);
-    launch_find_ids(d, ids, nc, idx, c->st);
-    std::vector<uint32_t> hidx(nc);
-    HIPCHK(c, hipMemcpyAsync(hidx.data(), idx, 4 * nc, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<uint32_t> found;
-    for (uint32_t k = 0; k < nc; ++k)
-      if (hidx[k] != 0xffffffffu) found.push_back(hidx[k]);
-    std::vector<double> hcm(8 * (size_t)nc, 0.0), hpos(3 * (size_t)nc, 0.0);
-    if (!found.empty()) {
-      HIPCHK(c, hipMemcpy(idx, found.data(), 4 * found.size(), hipMemcpyHostToDevice));
-      launch_neighbor_cm(d, idx, (uint32_t)found.size(), cm, c->st);
-      HIPCHK(c, hipStreamSynchronize(c->st));
-      HIPCHK(c, hipMemcpy(hcm.data(), cm, 64 * found.size(), hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < found.size(); ++k)
-        for (int a = 0; a < 3; ++a)
-          HIPCHK(c, hipMemcpy(&hpos[3 * k + a], d.pos + (size_t)a * d.cap + found[k], 8,
-                              hipMemcpyDeviceToHost));
-    }
-    std::vector<double> hcore(6 * (size_t)nc);
-    size_t fk = 0;
-    for (uint32_t k = 0; k < nc; ++k) {
-      double h[3] = {-1.0, -1.0, -1.0}, t[3] = {-1.0, -1.0, -1.0}, m[3] = {-1.0, -1.0, -1.0};
-      if (hidx[k] != 0xffffffffu) {
-        for (int a = 0; a < 3; ++a) {
-          h[a] = hpos[3 * fk + a];
-          m[a] = hcm[8 * fk + a];
-          t[a] = hcm[8 * fk + 3 + a];
-        }
-        ++fk;
-      }
-      double h2t[3] = {t[0] - h[0], t[1] - h[1], t[2] - h[2]};
-      min_image_round(box, h2t);
-      normalize(h2t);
-      double c2cm[3] = {m[0] - h[0], m[1] - h[1], m[2] - h[2]};
-      min_image_round(box, c2cm);
-      const double prj = c2cm[0] * h2t[0] + c2cm[1] * h2t[1] + c2cm[2] * h2t[2];
-      for (int a = 0; a < 3; ++a) h[a] += prj * h2t[a];
-      for (int a = 0; a < 3; ++a) h[a] -= std::floor(h[a] * (1.0 / box[a])) * box[a];
-      for (int a = 0; a < 3; ++a) {
-        hcore[6 * (size_t)k + a] = h[a];
-        hcore[6 * (size_t)k + 3 + a] = h2t[a];
-      }
-    }
-    HIPCHK(c, hipMemcpy(core, hcore.data(), 48 * (size_t)nc, hipMemcpyHostToDevice));
-    launch_chem_flags(d, core, nc, cp->p_thresld, cp->influ_rad, cp->influ_hei, cp->influ_dep,
-                      cp->influ_grd, c->time, flags, c->st);
-    std::vector<uint8_t> hf((size_t)nc * d.n_amp);
-    HIPCHK(c, hipMemcpyAsync(hf.data(), flags, hf.size(), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<uint32_t> topo(4 * (size_t)d.n_amp);
-    HIPCHK(c, hipMemcpy(topo.data(), d.topo, 16 * (size_t)d.n_amp, hipMemcpyDeviceToHost));
-    for (uint32_t k = 0; k < nc; ++k)
-      for (uint32_t j = 0; j < d.n_amp; ++j)
-        if (hf[(size_t)k * d.n_amp + j]) {
-          targets.push_back(topo[4 * (size_t)j]);
-          targets.push_back(topo[4 * (size_t)j + 2]);
-        }
-    hipFree(ids);
-    hipFree(idx);
-    hipFree(cm);
-    hipFree(core);
-    hipFree(flags);
+  ChemDev& h = c->chem_host[1];
+  h = ChemDev{};
+  h.p_thresld = cp->p_thresld;
+  h.eps = cp->eps;
+  h.influ_rad = cp->influ_rad;
+  h.influ_hei = cp->influ_hei;
+  h.influ_dep = cp->influ_dep;
+  h.influ_grd = cp->influ_grd;
+  h.bond_leng = c->P[0].bond_leng;
+  h.cf_b = c->P[0].cf_b;
+  h.max_amp_num = cp->max_amp_num;
+  h.n_core = cp->n_core;
+  h.in_run = 0;
+  for (uint32_t k = 0; k < cp->n_core; ++k) h.core_id[k] = cp->core_ptcl_id[k];
+  HIPCHK(c, hipMemcpyAsync(c->chem_dev + 1, &h, sizeof(ChemDev), hipMemcpyHostToDevice, c->st));
+  launch_fill_u32(d.d_time, 1, c->time, c->st);
+  // head and unit-2 positions of every chain, from the current positions
+  launch_chainpos_fill(d, c->st);
+  launch_chem_event(d, c->chem_dev + 1, c->chem_state + 1, cp->n_core, c->chem_counts,
+                    c->chem_targets, c->d_amp_num, false, c->st);
+  HIPCHK(c, hipGetLastError());
+  uint32_t res[5];
+  HIPCHK(c, hipMemcpyAsync(res, c->chem_state + 1, sizeof(res), hipMemcpyDeviceToHost, c->st));
+  if ((rc = pull_counts(c))) return rc;
+  if (res[4]) {  // refused: nothing was inserted
+    launch_fill_u32(d.flags, 1, 0u, c->st);
+    hipStreamSynchronize(c->st);
+    return fail(c, MDPD_ERR_CAPACITY, "inserted amphiphiles exceed the capacity");
   }
-  const uint32_t nt = (uint32_t)(targets.size() / 2);
-  if (nt) {
-    if (d.n + 4 * nt > c->cap || d.n_amp + nt > c->amp_cap)
-      return fail(c, MDPD_ERR_CAPACITY, "inserted amphiphiles exceed the capacity");
-    uint32_t* dt = nullptr;
-    HIPCHK(c, hipMalloc(&dt, 4 * targets.size()));
-    HIPCHK(c, hipMemcpy(dt, targets.data(), 4 * targets.size(), hipMemcpyHostToDevice));
-    launch_chem_insert(d, dt, nt, cp->eps, c->P[0].bond_leng, c->time, c->st);
-    std::vector<double> cfb(nt, c->P[0].cf_b);  // AppendCfBend, bdf_calculator.hpp:21-26
-    HIPCHK(c, hipMemcpyAsync(d.cf_bend + d.n_amp, cfb.data(), 8 * (size_t)nt, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    hipFree(dt);
-    d.n += 4 * nt;
-    d.n_amp += nt;
-    c->n_amp = d.n_amp;
-    c->amp_num_rep[0] += nt;
-    c->rep_off[1] = d.n;
-    c->R[0].n_beads = d.n;
-    c->amp_off[1] = d.n_amp;
-    HIPCHK(c, hipMemcpy(d.rep_off, c->rep_off.data(), 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(d.amp_off, c->amp_off.data(), 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_amp_num, c->amp_num_rep.data(), 4, hipMemcpyHostToDevice));
-    invalidate_graphs(c);
+  if (res[0]) {
     c->cells_fresh = false;
     c->have_cells = false;
     c->have_density = false;
@@ -313,6 +233,7 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
   d.n = n;
   d.n_amp = na;
   c->n_amp = na;
+  set_launch_bound(c);
   c->time = head[4];
   HIPCHK(c, hipMemcpy(d.d_time, &c->time, 4, hipMemcpyHostToDevice));
   c->drifted = false;

@@ -49,9 +49,9 @@ __global__ void __launch_bounds__(256) k_rdf(DevState d, const double* __restric
   for (int k = threadIdx.x; k < nb; k += blockDim.x) lh[k] = 0u;
   __syncthreads();
   const uint32_t s0 = blockIdx.x * blockDim.x;
-  const int r0 = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s0 < d.n ? s0 : d.n - 1) : 0;
+  const int r0 = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s0 < dev_n(d) ? s0 : dev_n(d) - 1) : 0;
   const uint32_t s = s0 + threadIdx.x;
-  if (s < d.n) {
+  if (s < dev_n(d)) {
     const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
     const Replica& R = d.reps[r];
     const float4 pi = d.spos[s];
@@ -91,10 +91,10 @@ __global__ void __launch_bounds__(256) k_rdf(DevState d, const double* __restric
 
 void launch_rdf(const DevState& d, const double* e2, int nb, double rmax,
                 unsigned long long* hist, hipStream_t st) {
-  if (!d.n) return;
+  if (!d.nl) return;
   // FP32 screen: the squared cutoff plus a margin far above the FP32 error
   const float cut2f = (float)(rmax * rmax) + 1e-4f;
-  hipLaunchKernelGGL(k_rdf, dim3(div_up(d.n, 256)), dim3(256), 0, st, d, e2, nb, cut2f, hist);
+  hipLaunchKernelGGL(k_rdf, dim3(div_up(d.nl, 256)), dim3(256), 0, st, d, e2, nb, cut2f, hist);
 }
 
 // ---------------------------------------------------------------- chains
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_count_map(DevState d, int g0, int g1, i
                                                    uint32_t type_mask,
                                                    unsigned int* __restrict__ map) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   if (!((type_mask >> (d.prop[i] & 31u)) & 1u)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(256) k_count_map(DevState d, int g0, int g1, i
 
 void launch_count_map(const DevState& d, int g0, int g1, int g2, uint32_t type_mask,
                       unsigned int* map, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_count_map, dim3(div_up(d.n, 256)), dim3(256), 0, st, d, g0, g1, g2,
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_count_map, dim3(div_up(d.nl, 256)), dim3(256), 0, st, d, g0, g1, g2,
                      type_mask, map);
 }
 

@@ -88,7 +88,7 @@ template <bool MULTI>
 __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
   if (*d.rebuild == 0u) return;
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
@@ -176,8 +176,9 @@ constexpr uint32_t kDensQ = 8;
 template <int ND, bool MULTI, bool RDLE1, int BS>
 __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
   __shared__ uint2 dens_queue[kDensQ * BS];  // [kDensQ][BS]
+  if (halted(d)) return;
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
@@ -321,7 +322,7 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
 template <int ND, bool MULTI>
 static void dens_launch(const DevState& d, int block, hipStream_t st) {
   // the queue lives in a static LDS array sized by the block
-  const dim3 g(div_up(d.n, block));
+  const dim3 g(div_up(d.nl, block));
   if (d.rd_le_1) {
     if (block == 64) hipLaunchKernelGGL((k_density_q<ND, MULTI, true, 64>), g, dim3(64), 0, st, d);
     else if (block == 128) hipLaunchKernelGGL((k_density_q<ND, MULTI, true, 128>), g, dim3(128), 0, st, d);
@@ -334,12 +335,12 @@ static void dens_launch(const DevState& d, int block, hipStream_t st) {
 }
 
 void launch_skin_build(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
+  if (!d.nl) return;
   const int block = 128;
   if (d.n_rep > 1)
-    hipLaunchKernelGGL((k_skin_build<true>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_skin_build<true>), dim3(div_up(d.nl, block)), dim3(block), 0, st, d);
   else
-    hipLaunchKernelGGL((k_skin_build<false>), dim3(div_up(d.n, block)), dim3(block), 0, st, d);
+    hipLaunchKernelGGL((k_skin_build<false>), dim3(div_up(d.nl, block)), dim3(block), 0, st, d);
 }
 
 // Density pass of skin mode: the same decisions and sums as k_density_q, over
@@ -355,7 +356,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
     d.rebuild[0] = 0u;
   }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
@@ -431,8 +432,8 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
 }
 
 void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st) {
-  if (!d.n) return;
-  const dim3 g(div_up(d.n, block)), b(block);
+  if (!d.nl) return;
+  const dim3 g(div_up(d.nl, block)), b(block);
   const bool multi = d.n_rep > 1;
   if (nd == 1) {
     if (multi) hipLaunchKernelGGL((k_density_skin<1, true>), g, b, 0, st, d);
@@ -444,7 +445,7 @@ void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st) {
 }
 
 void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_t st) {
-  if (!d.n) return;
+  if (!d.nl) return;
   const bool multi = d.n_rep > 1;
   if (nd == 1) multi ? dens_launch<1, true>(d, block, st) : dens_launch<1, false>(d, block, st);
   else multi ? dens_launch<3, true>(d, block, st) : dens_launch<3, false>(d, block, st);
@@ -478,6 +479,7 @@ __device__ __forceinline__ void fill_pair_table(const Replica& R, float4* tab) {
 template <int NS, int ND, int ENV, bool MULTI, bool VIRIAL>
 __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ vir_bead,
                                                int conv, int keys) {
+  if (halted(d)) return;
   // single system: the coefficients come from a block-shared table indexed
   // by the type pair instead of per-lane selects
   __shared__ float4 ptab[18];
@@ -496,7 +498,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     __syncthreads();
   }
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool active = s < d.n;
+  const bool active = s < dev_n(d);
   // with keys every lane of the wave reaches the aggregated histogram
   // increment at the end
   if (!active && !keys) return;
@@ -676,7 +678,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
 template <int NS, int ND, int ENV>
 static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb,
                           int cv, int ky) {
-  const dim3 g(div_up(d.n, block)), b(block);
+  const dim3 g(div_up(d.nl, block)), b(block);
   const bool multi = d.n_rep > 1;
   if (multi) {
     if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv, ky);
@@ -696,7 +698,7 @@ static void force_launch2(const DevState& d, int env, bool virial, int block, hi
 
 void launch_force(const DevState& d, int ns, int nd, int env, bool virial, int block,
                   hipStream_t st, bool fuse, bool conv, bool keys) {
-  if (!d.n) return;
+  if (!d.nl) return;
   double* vb = fuse ? bead_vir_buffer(d) : nullptr;
   const int cv = (fuse && conv) ? 1 : 0;
   const int ky = (fuse && keys) ? 1 : 0;
@@ -714,7 +716,7 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
                                                     unsigned long long* __restrict__ counter,
                                                     uint64_t cap) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const float4 pi = d.spos[s];
@@ -748,8 +750,8 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
 
 void launch_pairs_list(const DevState& d, double cutoff, uint32_t* out_i, uint32_t* out_j,
                        unsigned long long* counter, uint64_t cap, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_pairs_list, dim3(div_up(d.n, 256)), dim3(256), 0, st, d,
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_pairs_list, dim3(div_up(d.nl, 256)), dim3(256), 0, st, d,
                      (float)(cutoff * cutoff), cutoff * cutoff, out_i, out_j, counter, cap);
 }
 

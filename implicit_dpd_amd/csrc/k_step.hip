@@ -51,12 +51,13 @@ __global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
 // it and writes the new state once.
 __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict_only, int tick,
                                                int store) {
+  if (halted(d)) return;
   // tick: the step's clock advances here (no kernel of the step before
   // this one reads it)
   if (tick && blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i0 < d.n;
-  const uint32_t i = active ? i0 : d.n - 1;
+  const bool active = i0 < dev_n(d);
+  const uint32_t i = active ? i0 : dev_n(d) - 1;
   uint32_t k = 0;
   if (active) {
     const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
@@ -92,19 +93,19 @@ __global__ void __launch_bounds__(256) k_drift(DevState d, int keys, int predict
   if (keys) (void)agg_inc(d.cell_cnt, k, active);
 }
 void launch_drift(const DevState& d, bool keys, bool predict_only, hipStream_t st, bool tick) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, keys ? 1 : 0,
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_drift, grid1(d.nl, 256), dim3(256), 0, st, d, keys ? 1 : 0,
                      predict_only ? 1 : 0, tick ? 1 : 0, 1);
 }
 void launch_step_keys(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_drift, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 0, 0);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_drift, grid1(d.nl, 256), dim3(256), 0, st, d, 1, 0, 0, 0);
 }
 
 __global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
   if (gate && *d.rebuild == 0u) return;
   const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i0 < d.n;
+  const bool active = i0 < dev_n(d);
   uint32_t k = 0;
   if (active) {
     const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i0) : 0;
@@ -117,8 +118,8 @@ __global__ void __launch_bounds__(256) k_keys(DevState d, int gate) {
   (void)agg_inc(d.cell_cnt, k, active);
 }
 void launch_keys(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d, 0);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_keys, grid1(d.nl, 256), dim3(256), 0, st, d, 0);
 }
 
 // exclusive scan of the cell histogram (1024 cells per block, three passes)
@@ -173,7 +174,9 @@ __global__ void __launch_bounds__(256) k_scan1(uint32_t* __restrict__ in,
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint32_t nb) {
+__global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint32_t nb,
+                                               const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   uint32_t carry = 0;
   for (uint32_t base = 0; base < nb; base += 1024u) {
     const uint32_t b0 = base + threadIdx.x * 4u;
@@ -200,7 +203,8 @@ __global__ void __launch_bounds__(256) k_scan2(uint32_t* __restrict__ bsum, uint
 __global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
                                                uint32_t* __restrict__ cursor,
                                                const uint32_t* __restrict__ bsum, uint32_t m,
-                                               uint32_t total, int carry,
+                                               uint32_t total,
+                                               const uint32_t* __restrict__ total_dev, int carry,
                                                const uint32_t* __restrict__ gate) {
   if (gate && *gate == 0u) return;
   const uint32_t base = blockIdx.x * 1024u;
@@ -225,7 +229,7 @@ __global__ void __launch_bounds__(256) k_scan3(uint32_t* __restrict__ out,
       cursor[idx] = v;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total_dev ? *total_dev : total;
 }
 
 static void scan_cells(const DevState& d, const uint32_t* gate, hipStream_t st) {
@@ -234,23 +238,27 @@ static void scan_cells(const DevState& d, const uint32_t* gate, hipStream_t st) 
   hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, st, d.cell_cnt, d.cell_start, d.bsum, m,
                      gate);
   const int carry = nb <= 4096u ? 1 : 0;  // up to 4M cells: no separate block-sum pass
-  if (!carry) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb);
+  if (!carry) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(256), 0, st, d.bsum, nb, gate);
+  // in grow mode the bead count is the device's (chemistry may have added beads)
+  const uint32_t* total_dev = d.run_ok ? d.rep_off + d.n_rep : nullptr;
   hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, st, d.cell_start, d.cursor, d.bsum, m, d.n,
-                     carry, gate);
+                     total_dev, carry, gate);
 }
-void launch_scan_cells(const DevState& d, hipStream_t st) { scan_cells(d, nullptr, st); }
+// (grow mode: a run stopped by the chemistry skips its remaining steps)
+void launch_scan_cells(const DevState& d, hipStream_t st) { scan_cells(d, d.run_ok, st); }
 
 __global__ void __launch_bounds__(256) k_scatter(DevState d, int gate) {
   if (gate && *d.rebuild == 0u) return;
+  if (halted(d)) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < d.n;
+  const bool active = i < dev_n(d);
   const uint32_t k = active ? d.key[i] : 0u;
   const uint32_t slot = agg_inc(d.cursor, k, active);
   if (active) d.perm[slot] = i;
 }
 void launch_scatter(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d, 0);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_scatter, grid1(d.nl, 256), dim3(256), 0, st, d, 0);
 }
 
 // ascending bead slots inside each cell, so the sorted order (and every FP32
@@ -286,7 +294,7 @@ void launch_cell_sort(const DevState& d, hipStream_t st) {
 __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int full, int skin) {
   if (skin && *d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const uint32_t i = d.perm[s];
   if (full) d.inv_perm[i] = s;  // for the explicit rebuild's topology remap
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -333,8 +341,8 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
   d.scell[s] = key;
 }
 void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_permute, grid1(d.nl, 256), dim3(256), 0, st, d, predicted ? 1 : 0,
                      full ? 1 : 0, 0);
 }
 
@@ -346,9 +354,10 @@ void launch_permute(const DevState& d, bool predicted, bool full, hipStream_t st
 // replica constants are wave-uniform.  Same arithmetic as k_permute.
 template <bool ST, bool MULTI>
 __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
+  if (halted(d)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const uint32_t i = d.perm[s];
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
@@ -418,9 +427,9 @@ __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
 }
 
 void launch_step_permute(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  const bool big = d.n >= kStreamMin, multi = d.n_rep > 1;
-  const dim3 g = grid1(d.n, 256), b(256);
+  if (!d.nl) return;
+  const bool big = d.nl >= kStreamMin, multi = d.n_rep > 1;
+  const dim3 g = grid1(d.nl, 256), b(256);
   if (big) {
     if (multi) hipLaunchKernelGGL((k_step_permute<true, true>), g, b, 0, st, d);
     else hipLaunchKernelGGL((k_step_permute<true, false>), g, b, 0, st, d);
@@ -444,13 +453,13 @@ void launch_topo_remap(const DevState& d, hipStream_t st) {
 // the topology from the chain map (every chain bead knows its chain slot)
 __global__ void __launch_bounds__(256) k_topo_from_chain(DevState d) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const uint32_t ch = d.chain[s];
   if (ch != 0xffffffffu) d.topo[ch] = s;
 }
 void launch_topo_from_chain(const DevState& d, hipStream_t st) {
-  if (!d.n || !d.n_amp) return;
-  hipLaunchKernelGGL(k_topo_from_chain, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl || !d.n_amp) return;
+  hipLaunchKernelGGL(k_topo_from_chain, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 // ---------------------------------------------------------------- kick
@@ -458,7 +467,7 @@ void launch_topo_from_chain(const DevState& d, hipStream_t st) {
 __global__ void __launch_bounds__(256) k_kick(DevState d) {
 #pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const double dt = d.reps[r].dt;
   const uint32_t cap = d.cap;
@@ -467,8 +476,8 @@ __global__ void __launch_bounds__(256) k_kick(DevState d) {
     d.vel[a * cap + i] = d.vbuf[a * cap + i] + dt * 0.5 * (double)d.acc[a * cap + i];
 }
 void launch_kick(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_kick, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_kick, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -523,7 +532,9 @@ __device__ __forceinline__ void block_reduce_write(KSum (&acc)[Q], double* __res
 
 // final: one block per replica sums its kRedBlocks partials in order
 template <int Q>
-__global__ void k_red_final(const double* __restrict__ part, double* __restrict__ out) {
+__global__ void k_red_final(const double* __restrict__ part, double* __restrict__ out,
+                            const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   const int r = blockIdx.x;
   if (threadIdx.x < Q) {
     const int q = threadIdx.x;
@@ -538,6 +549,7 @@ __global__ void k_red_final(const double* __restrict__ part, double* __restrict_
 
 // sum of velocities per replica (for remove_cmdrift_global)
 __global__ void __launch_bounds__(kRedThreads) k_cm_sum(DevState d) {
+  if (halted(d)) return;
   const int r = blockIdx.y;
   const uint32_t b0 = d.rep_off[r], n_r = d.rep_off[r + 1] - b0;
   KSum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
@@ -551,8 +563,9 @@ __global__ void __launch_bounds__(kRedThreads) k_cm_sum(DevState d) {
 
 __global__ void __launch_bounds__(256) k_cm_apply(DevState d) {
 #pragma clang fp contract(off)
+  if (halted(d)) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const double n_r = (double)(d.rep_off[r + 1] - d.rep_off[r]);
 #pragma unroll
@@ -560,10 +573,11 @@ __global__ void __launch_bounds__(256) k_cm_apply(DevState d) {
 }
 
 void launch_cm_drift(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
+  if (!d.nl) return;
   hipLaunchKernelGGL(k_cm_sum, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d);
-  hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, d.red_out);
-  hipLaunchKernelGGL(k_cm_apply, grid1(d.n, 256), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, d.red_out,
+                     (const uint32_t*)d.run_ok);
+  hipLaunchKernelGGL(k_cm_apply, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 // Observer sums per replica (observer.hpp:82-178); amp_num[r] is the
@@ -575,7 +589,7 @@ void launch_cm_drift(const DevState& d, hipStream_t st) {
 //  pair-list entries (sorted slots of the replica).
 __global__ void __launch_bounds__(256) k_inv_orig(DevState d, uint32_t* __restrict__ inv_orig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.n) inv_orig[d.orig[i]] = i;
+  if (i < dev_n(d)) inv_orig[d.orig[i]] = i;
 }
 
 __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
@@ -628,10 +642,11 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
 
 void launch_observe(const DevState& d, const uint32_t* amp_num, uint32_t* inv_orig, double* out,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_inv_orig, grid1(d.n, 256), dim3(256), 0, st, d, inv_orig);
+  hipLaunchKernelGGL(k_inv_orig, grid1(d.nl, 256), dim3(256), 0, st, d, inv_orig);
   hipLaunchKernelGGL(k_observe, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d, amp_num,
                      (const uint32_t*)inv_orig, d.red);
-  hipLaunchKernelGGL(k_red_final<13>, dim3(d.n_rep), dim3(64), 0, st, d.red, out);
+  hipLaunchKernelGGL(k_red_final<13>, dim3(d.n_rep), dim3(64), 0, st, d.red, out,
+                     (const uint32_t*)nullptr);
 }
 
 // ---------------------------------------------------------------- bonded
@@ -642,7 +657,7 @@ void launch_observe(const DevState& d, const uint32_t* amp_num, uint32_t* inv_or
 __global__ void __launch_bounds__(256) k_bonded_kick(DevState d, int virial, int kick,
                                                      double* __restrict__ vir_bead) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, s) : 0;
   float af[3];
 #pragma unroll
@@ -654,19 +669,20 @@ __global__ void __launch_bounds__(256) k_bonded_kick(DevState d, int virial, int
 // (the step's permutation writes them as it moves the beads)
 __global__ void __launch_bounds__(256) k_chainpos_fill(DevState d) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const uint32_t ch = d.chain[s];
   if (ch != 0xffffffffu)
     d.chainpos[ch] = make_double4(d.pos[s], d.pos[d.cap + s], d.pos[2 * (size_t)d.cap + s], 0.0);
 }
 void launch_chainpos_fill(const DevState& d, hipStream_t st) {
-  if (!d.n || !d.n_amp) return;
-  hipLaunchKernelGGL(k_chainpos_fill, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl || !d.n_amp) return;
+  hipLaunchKernelGGL(k_chainpos_fill, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 // per-replica sum of the per-bead chain virials
 __global__ void __launch_bounds__(kRedThreads) k_bead_vir(DevState d, const double* __restrict__ vb,
                                                           double* __restrict__ part) {
+  if (halted(d)) return;
   const int r = blockIdx.y;
   const uint32_t b0 = d.rep_off[r], nr = d.rep_off[r + 1] - b0;
   KSum acc[3] = {{0, 0}, {0, 0}, {0, 0}};
@@ -684,7 +700,8 @@ void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st) 
   if (d.n_amp) {
     hipLaunchKernelGGL(k_bead_vir, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d,
                        (const double*)bead_vir_buffer(d), d.red);
-    hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out);
+    hipLaunchKernelGGL(k_red_final<3>, dim3(d.n_rep), dim3(64), 0, st, d.red, vir_out,
+                       (const uint32_t*)d.run_ok);
   } else {
     hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(256), 0, st, vir_out, (size_t)3 * d.n_rep, 0.0);
   }
@@ -692,9 +709,9 @@ void launch_bead_vir_reduce(const DevState& d, double* vir_out, hipStream_t st) 
 
 void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_out,
                         hipStream_t st) {
-  if (!d.n) return;
+  if (!d.nl) return;
   if (d.n_amp || kick)
-    hipLaunchKernelGGL(k_bonded_kick, grid1(d.n, 256), dim3(256), 0, st, d, virial ? 1 : 0,
+    hipLaunchKernelGGL(k_bonded_kick, grid1(d.nl, 256), dim3(256), 0, st, d, virial ? 1 : 0,
                        kick ? 1 : 0, bead_vir_buffer(d));
   if (virial) launch_bead_vir_reduce(d, vir_out, st);
 }
@@ -704,7 +721,7 @@ void launch_bonded_kick(const DevState& d, bool virial, bool kick, double* vir_o
 // Saru(amp_id, time, chem_seed).uni_open() <= conv_p
 __global__ void __launch_bounds__(256) k_head_conv(DevState d) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   if (d.unit[i] != 0u) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
@@ -720,8 +737,8 @@ __global__ void __launch_bounds__(256) k_head_conv(DevState d) {
   }
 }
 void launch_head_conversion(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_head_conv, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_head_conv, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 
@@ -736,7 +753,7 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
 #pragma clang fp contract(off)
   if (blockIdx.x == 0 && threadIdx.x == 0) *d.d_time += 1u;  // the step's clock
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
   const uint32_t cap = d.cap;
@@ -778,25 +795,25 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
   if (ch != 0xffffffffu) d.chainpos[ch] = make_double4(p[0], p[1], p[2], 0.0);
 }
 void launch_drift_skin(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_drift_skin, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_drift_skin, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 void launch_keys_gated(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_keys, grid1(d.n, 256), dim3(256), 0, st, d, 1);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_keys, grid1(d.nl, 256), dim3(256), 0, st, d, 1);
 }
 void launch_scan_cells_gated(const DevState& d, hipStream_t st) { scan_cells(d, d.rebuild, st); }
 void launch_scatter_gated(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_scatter, grid1(d.n, 256), dim3(256), 0, st, d, 1);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_scatter, grid1(d.nl, 256), dim3(256), 0, st, d, 1);
 }
 void launch_cell_sort_gated(const DevState& d, hipStream_t st) {
   hipLaunchKernelGGL(k_cell_sort, grid1(d.ncell, 256), dim3(256), 0, st, d, 1);
 }
 void launch_permute_gated(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_permute, grid1(d.n, 256), dim3(256), 0, st, d, 1, 0, 1);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_permute, grid1(d.nl, 256), dim3(256), 0, st, d, 1, 0, 1);
 }
 
 // The rebuild's permutation wrote the other buffer set; copy it back so the
@@ -806,7 +823,7 @@ void launch_permute_gated(const DevState& d, hipStream_t st) {
 __global__ void __launch_bounds__(256) k_copy_back(DevState d) {
   if (*d.rebuild == 0u) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.n) return;
+  if (s >= dev_n(d)) return;
   const uint32_t cap = d.cap;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -822,14 +839,14 @@ __global__ void __launch_bounds__(256) k_copy_back(DevState d) {
   d.chain[s] = d.alt.chain[s];
 }
 void launch_copy_back_gated(const DevState& d, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_copy_back, grid1(d.n, 256), dim3(256), 0, st, d);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_copy_back, grid1(d.nl, 256), dim3(256), 0, st, d);
 }
 
 // bead counts along one axis per replica
 __global__ void __launch_bounds__(256) k_profile(DevState d, int axis, int nbins, uint32_t* hist) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
   int b = (int)floor(d.pos[axis * d.cap + i] * R.inv_box[axis] * nbins);
@@ -837,8 +854,8 @@ __global__ void __launch_bounds__(256) k_profile(DevState d, int axis, int nbins
   atomicAdd(&hist[(size_t)r * nbins + b], 1u);
 }
 void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_profile, grid1(d.n, 256), dim3(256), 0, st, d, axis, nbins, hist);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_profile, grid1(d.nl, 256), dim3(256), 0, st, d, axis, nbins, hist);
 }
 
 // ---------------------------------------------------------------- neighbour CM
@@ -859,12 +876,9 @@ __device__ __forceinline__ double fp64_disp(const DevState& d, const Replica& R,
   return dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
 }
 
-__global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, uint32_t nb,
-                              double* __restrict__ out) {
+// out[0..2] hydrophilic CM, out[3..5] hydrophobic CM, out[6..7] counts
+__device__ void neighbor_cm_at(const DevState& d, uint32_t i, double out[8]) {
 #pragma clang fp contract(off)
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nb) return;
-  const uint32_t i = beads[k];
   const int r = d.n_rep > 1 ? find_rep(d.rep_off, d.n_rep, i) : 0;
   const Replica& R = d.reps[r];
   double ri[3], hl[3], hb[3] = {0.0, 0.0, 0.0};
@@ -903,11 +917,20 @@ __global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, ui
         }
   }
   for (int a = 0; a < 3; ++a) {
-    out[8 * (size_t)k + a] = hl[a] / (double)cl;
-    out[8 * (size_t)k + 3 + a] = hb[a] / (double)cb;
+    out[a] = hl[a] / (double)cl;
+    out[3 + a] = hb[a] / (double)cb;
   }
-  out[8 * (size_t)k + 6] = (double)cl;
-  out[8 * (size_t)k + 7] = (double)cb;
+  out[6] = (double)cl;
+  out[7] = (double)cb;
+}
+
+__global__ void k_neighbor_cm(DevState d, const uint32_t* __restrict__ beads, uint32_t nb,
+                              double* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  double o[8];
+  neighbor_cm_at(d, beads[k], o);
+  for (int q = 0; q < 8; ++q) out[8 * (size_t)k + q] = o[q];
 }
 void launch_neighbor_cm(const DevState& d, const uint32_t* beads, uint32_t nb, double* out,
                         hipStream_t st) {
@@ -932,118 +955,286 @@ void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3
 }
 
 // ---------------------------------------------------------------- chemistry
-__global__ void k_find_ids(DevState d, const uint32_t* ids, uint32_t nid, uint32_t* idx) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  const uint32_t v = d.id[i];
-  for (uint32_t k = 0; k < nid; ++k)
-    if (v == ids[k]) idx[k] = i;
-}
-void launch_find_ids(const DevState& d, const uint32_t* ids, uint32_t nid, uint32_t* idx,
-                     hipStream_t st) {
-  if (!d.n || !nid) return;
-  hipLaunchKernelGGL(k_find_ids, grid1(d.n, 256), dim3(256), 0, st, d, ids, nid, idx);
+// ChemManager::RandomChemEvent, serial version (chemmanager.hpp:244-281),
+// with DetermineTargetIdWithCorePos (:116-164), GetCorePos (:84-114),
+// GetMembraneTangRandomVec (:166-173) and CreateAmpPart (:35-67), entirely
+// on the device so that it can sit inside a captured step:
+//   k_chem_find    storage slot of every core head (bead ids in cd->core_id)
+//   k_chem_core    per core: neighbour CMs, membrane normal h2t, the head
+//                  moved along h2t by the projection of its hydrophilic CM
+//   k_chem_count   per (core, block of 256 amphiphiles): how many heads lie
+//                  in the truncated cone and draw <= p_thresld
+//   k_chem_scan    one block: exclusive offsets of those counts in (core,
+//                  amphiphile) order, the capacity check, the stop test
+//   k_chem_scatter the targets' amphiphile indices at those offsets
+//   k_chem_insert  four new beads per target, appended at n0 + 4t + u
+//   k_chem_commit  the new bead and chain counts (rep_off / amp_off on the
+//                  device), the stop of the run, the core slots reset
+// Head and unit-2 positions come from the chain-major FP64 copy (chainpos,
+// written by every permutation), so no topology pass is needed.  Draws are
+// keyed Saru streams: the cone test (amphiphile j, time, chem_seed +
+// TAG_CONE (core + 1)), the tangent angle (new amphiphile id, time,
+// chem_seed ^ TAG_TANG), the velocities (new bead id, time, chem_seed ^
+// TAG_VEL).  `time` is the device clock.
+constexpr int kChemBlock = 256;
+
+__global__ void __launch_bounds__(256) k_chem_find(DevState d, const ChemDev* __restrict__ cd,
+                                                   ChemState* __restrict__ cs) {
+  if (halted(d)) return;
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= dev_n(d)) return;
+  const uint32_t id = d.id[s], nc = cd->n_core;
+  for (uint32_t c = 0; c < nc; ++c)
+    if (cd->core_id[c] == id) cs->core_slot[c] = s;
 }
 
-// DetermineTargetIdWithCorePos loop body (chemmanager.hpp:140-161) for every
-// amphiphile j and core c; core[c] = (corrected head position, h2t).
-__global__ void k_chem_flags(DevState d, const double* __restrict__ core, uint32_t n_core,
-                             double p_thresld, double influ_rad, double influ_hei,
-                             double influ_dep, double influ_grd, uint32_t time,
-                             uint8_t* __restrict__ flags) {
+__global__ void __launch_bounds__(256) k_chem_core(DevState d, const ChemDev* __restrict__ cd,
+                                                   ChemState* __restrict__ cs) {
 #pragma clang fp contract(off)
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t c = blockIdx.y;
-  if (j >= d.n_amp || c >= n_core) return;
+  if (halted(d)) return;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cd->n_core) return;
   const Replica& R = d.reps[0];
-  const double* ch = core + 6 * (size_t)c;
+  // GetCorePos: (-1, -1, -1) when the core head is absent
+  double h[3] = {-1.0, -1.0, -1.0}, t[3] = {-1.0, -1.0, -1.0}, m[3] = {-1.0, -1.0, -1.0};
+  const uint32_t slot = cs->core_slot[c];
+  if (slot != 0xffffffffu) {
+    double o[8];
+    neighbor_cm_at(d, slot, o);
+    for (int a = 0; a < 3; ++a) {
+      h[a] = d.pos[a * d.cap + slot];
+      m[a] = o[a];
+      t[a] = o[3 + a];
+    }
+  }
+  double h2t[3] = {t[0] - h[0], t[1] - h[1], t[2] - h[2]};
+  bd_min_image(R, h2t);
+  double l = sqrt(h2t[0] * h2t[0] + h2t[1] * h2t[1] + h2t[2] * h2t[2]);
+  for (int a = 0; a < 3; ++a) h2t[a] /= l;
+  double c2cm[3] = {m[0] - h[0], m[1] - h[1], m[2] - h[2]};
+  bd_min_image(R, c2cm);
+  const double prj = c2cm[0] * h2t[0] + c2cm[1] * h2t[1] + c2cm[2] * h2t[2];
+  for (int a = 0; a < 3; ++a) h[a] += prj * h2t[a];
+  for (int a = 0; a < 3; ++a) h[a] -= floor(h[a] * R.inv_box[a]) * R.box[a];
+  for (int a = 0; a < 3; ++a) {
+    cs->core_geo[c][a] = h[a];
+    cs->core_geo[c][3 + a] = h2t[a];
+  }
+}
+
+// DetermineTargetIdWithCorePos loop body (chemmanager.hpp:140-161)
+__device__ __forceinline__ bool chem_target(const DevState& d, const ChemDev* __restrict__ cd,
+                                            const ChemState* __restrict__ cs, uint32_t c,
+                                            uint32_t j, uint32_t time) {
+#pragma clang fp contract(off)
+  const Replica& R = d.reps[0];
+  const double* ch = cs->core_geo[c];
   const double* h2t = ch + 3;
-  const uint32_t head = d.topo[4 * j];
-  double c2p[3];
-  for (int a = 0; a < 3; ++a) c2p[a] = d.pos[a * d.cap + head] - ch[a];
+  const double4 hp = d.chainpos[4 * (size_t)j];
+  double c2p[3] = {hp.x - ch[0], hp.y - ch[1], hp.z - ch[2]};
   bd_min_image(R, c2p);
   const double depth = h2t[0] * c2p[0] + h2t[1] * c2p[1] + h2t[2] * c2p[2];
   double tg[3];
   for (int a = 0; a < 3; ++a) tg[a] = c2p[a] - depth * h2t[a];
   const double rad = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
-  const double rad_thr = influ_grd * depth + influ_rad;
+  const double rad_thr = cd->influ_grd * depth + cd->influ_rad;
   double rnd = 2.0;
-  if ((rad <= rad_thr) && (depth >= -influ_hei) && (depth <= influ_dep)) {
+  if ((rad <= rad_thr) && (depth >= -cd->influ_hei) && (depth <= cd->influ_dep)) {
     Saru s = saru_seed(j, time, R.chem_seed + TAG_CONE * (c + 1u));
     rnd = saru_uni_f64(s);
   }
-  flags[(size_t)c * d.n_amp + j] = (rnd <= p_thresld) ? 1 : 0;
-}
-void launch_chem_flags(const DevState& d, const double* core, uint32_t n_core, double p_thresld,
-                       double influ_rad, double influ_hei, double influ_dep, double influ_grd,
-                       uint32_t time, uint8_t* flags, hipStream_t st) {
-  if (!d.n_amp || !n_core) return;
-  hipLaunchKernelGGL(k_chem_flags, dim3(div_up(d.n_amp, 256), n_core), dim3(256), 0, st, d, core,
-                     n_core, p_thresld, influ_rad, influ_hei, influ_dep, influ_grd, time, flags);
+  return rnd <= cd->p_thresld;
 }
 
-// RandomChemEvent insertion loop (chemmanager.hpp:258-271) with CreateAmpPart
-// (:35-67): target t = (head slot, tail slot) makes amphiphile n_amp + t of
-// beads n + 4t + u.  One thread per new bead.
-__global__ void k_chem_insert(DevState d, const uint32_t* __restrict__ targets, uint32_t ntarget,
-                              double eps, double bond_leng, uint32_t time) {
+__global__ void __launch_bounds__(kChemBlock) k_chem_count(DevState d, const ChemDev* __restrict__ cd,
+                                                           const ChemState* __restrict__ cs,
+                                                           uint32_t* __restrict__ counts) {
+  if (halted(d)) return;
+  const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
+  const bool f = j < dev_namp(d) && chem_target(d, cd, cs, c, j, *d.d_time);
+  const int cnt = __syncthreads_count(f ? 1 : 0);
+  if (threadIdx.x == 0) counts[(size_t)c * gridDim.x + blockIdx.x] = (uint32_t)cnt;
+}
+
+// one block of 1024 threads: counts[0..m) -> exclusive offsets, in order
+__global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* __restrict__ cd,
+                                                    ChemState* __restrict__ cs,
+                                                    uint32_t* __restrict__ counts, uint32_t m) {
+  if (halted(d)) return;
+  __shared__ uint32_t wsum[16];
+  const uint32_t per = (m + 1023u) / 1024u;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(b0 + per, m);
+  uint32_t tsum = 0;
+  for (uint32_t k = b0; k < b1; ++k) tsum += counts[k];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(tsum);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int k = 0; k < 16; ++k) {
+    if (k < w) off += wsum[k];
+    tot += wsum[k];
+  }
+  uint32_t run = off + inc - tsum;
+  for (uint32_t k = b0; k < b1; ++k) {
+    const uint32_t v = counts[k];
+    counts[k] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t n0 = dev_n(d), na0 = dev_namp(d);
+    // amp_cap = cap / 4 + 1 chains fit the chain arrays
+    const bool fits = (uint64_t)n0 + 4ull * tot <= (uint64_t)d.cap &&
+                      (uint64_t)na0 + tot <= (uint64_t)d.cap / 4u + 1u;
+    cs->n0 = n0;
+    cs->namp0 = na0;
+    cs->nt = fits ? tot : 0u;
+    cs->refused = fits ? 0u : 1u;
+    cs->stop = (fits && na0 + tot >= cd->max_amp_num) ? 1u : 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kChemBlock) k_chem_scatter(DevState d, const ChemDev* __restrict__ cd,
+                                                             const ChemState* __restrict__ cs,
+                                                             const uint32_t* __restrict__ offs,
+                                                             uint32_t* __restrict__ targets) {
+  if (halted(d) || cs->nt == 0u) return;
+  __shared__ uint32_t wcnt[kChemBlock / 64];
+  const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
+  const bool f = j < dev_namp(d) && chem_target(d, cd, cs, c, j, *d.d_time);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(f);
+  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t base = offs[(size_t)c * gridDim.x + blockIdx.x];
+  for (int k = 0; k < w; ++k) base += wcnt[k];
+  if (f) targets[base + below] = j;
+}
+
+// one thread per new bead (grid-stride): target t makes amphiphile namp0 + t
+// of beads n0 + 4t + u.  With `keys` (this step's force pass binned every
+// other bead for the next step) the new bead's cell key and histogram entry
+// of its next drift are added too; a run that stops here clears the
+// histogram instead (the next run bins every bead itself).
+__global__ void __launch_bounds__(256) k_chem_insert(DevState d, const ChemDev* __restrict__ cd,
+                                                     const ChemState* __restrict__ cs,
+                                                     const uint32_t* __restrict__ targets,
+                                                     int keys) {
 #pragma clang fp contract(off)
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= 4 * ntarget) return;
-  const uint32_t t = g >> 2, u = g & 3u;
+  if (halted(d)) return;
+  const uint32_t nt = cs->nt, n0 = cs->n0, na0 = cs->namp0;
+  const bool stop_run = cd->in_run && (cs->stop || cs->refused);
+  const uint32_t gsz = gridDim.x * blockDim.x;
+  const uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (stop_run && keys)
+    for (uint32_t k = g0; k < d.ncell; k += gsz) d.cell_cnt[k] = 0u;
   const Replica& R = d.reps[0];
-  const uint32_t hd = targets[2 * t], tl = targets[2 * t + 1];
   const uint32_t cap = d.cap;
-  double h2e[3];
-  for (int a = 0; a < 3; ++a) h2e[a] = d.pos[a * cap + tl] - d.pos[a * cap + hd];
-  bd_min_image(R, h2e);
-  double l = sqrt(h2e[0] * h2e[0] + h2e[1] * h2e[1] + h2e[2] * h2e[2]);
-  for (int a = 0; a < 3; ++a) h2e[a] /= l;
-  const uint32_t new_amp = d.n_amp + t;
-  Saru st = saru_seed(new_amp, time, R.chem_seed ^ TAG_TANG);
-  const double theta = 2.0 * M_PI * saru_uni_f64(st);
-  double base[3] = {cos(theta), sin(theta), 0.0};
-  const double bd = base[0] * h2e[0] + base[1] * h2e[1] + base[2] * h2e[2];
-  double tv[3];
-  for (int a = 0; a < 3; ++a) tv[a] = base[a] - h2e[a] * bd;
-  l = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
-  for (int a = 0; a < 3; ++a) tv[a] /= l;
-  const uint32_t i = d.n + g;
-  double np[3];
-  for (int a = 0; a < 3; ++a) {
-    tv[a] *= eps;
-    h2e[a] *= bond_leng;
-    double x = (tv[a] + d.pos[a * cap + hd]) + h2e[a] * (double)u;
-    x -= floor(x * R.inv_box[a]) * R.box[a];
-    d.pos[a * cap + i] = x;
-    np[a] = x;
+  const uint32_t time = *d.d_time;
+  for (uint32_t g = g0; g < 4u * nt; g += gsz) {
+    const uint32_t t = g >> 2, u = g & 3u;
+    const uint32_t j = targets[t];
+    const double4 hp = d.chainpos[4 * (size_t)j], tp = d.chainpos[4 * (size_t)j + 2];
+    const double hd[3] = {hp.x, hp.y, hp.z};
+    double h2e[3] = {tp.x - hp.x, tp.y - hp.y, tp.z - hp.z};
+    bd_min_image(R, h2e);
+    double l = sqrt(h2e[0] * h2e[0] + h2e[1] * h2e[1] + h2e[2] * h2e[2]);
+    for (int a = 0; a < 3; ++a) h2e[a] /= l;
+    const uint32_t new_amp = na0 + t;
+    Saru st = saru_seed(new_amp, time, R.chem_seed ^ TAG_TANG);
+    const double theta = 2.0 * M_PI * saru_uni_f64(st);
+    double base[3] = {cos(theta), sin(theta), 0.0};
+    const double bd = base[0] * h2e[0] + base[1] * h2e[1] + base[2] * h2e[2];
+    double tv[3];
+    for (int a = 0; a < 3; ++a) tv[a] = base[a] - h2e[a] * bd;
+    l = sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+    for (int a = 0; a < 3; ++a) tv[a] /= l;
+    const uint32_t i = n0 + g;
+    double np[3], vv[3];
+    for (int a = 0; a < 3; ++a) {
+      tv[a] *= cd->eps;
+      h2e[a] *= cd->bond_leng;
+      double x = (tv[a] + hd[a]) + h2e[a] * (double)u;
+      x -= floor(x * R.inv_box[a]) * R.box[a];
+      d.pos[a * cap + i] = x;
+      np[a] = x;
+    }
+    d.chainpos[4 * (size_t)new_amp + u] = make_double4(np[0], np[1], np[2], 0.0);
+    Saru sv = saru_seed(i, time, R.chem_seed ^ TAG_VEL);
+    for (int a = 0; a < 3; ++a) {
+      const double u1 = saru_uni_f64(sv);
+      const double u2 = saru_uni_f64(sv);
+      vv[a] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+      d.vel[a * cap + i] = vv[a];
+      d.vbuf[a * cap + i] = 0.0;
+      d.acc[a * cap + i] = 0.0f;
+      d.press[a * cap + i] = 0.0f;
+      d.delta[a * cap + i] = 0.0;
+    }
+    d.id[i] = i;
+    d.orig[i] = i;
+    d.prop[i] = (u == 0) ? 0u : 1u;
+    d.amp_id[i] = new_amp;
+    d.unit[i] = u;
+    d.topo[4 * (size_t)new_amp + u] = i;
+    d.chain[i] = (new_amp << 2) | u;
+    if (u == 0) {
+      d.amp_rep[new_amp] = 0u;
+      d.cf_bend[new_amp] = cd->cf_b;  // AppendCfBend, bdf_calculator.hpp:21-26
+    }
+    if (keys && !stop_run) {
+      // the next step's drift of the new bead (acc = 0), as k_drift bins it
+      double p[3];
+      for (int a = 0; a < 3; ++a) p[a] = drift_axis(R, a, np[a], vv[a], 0.0, 0.0).p;
+      atomicAdd(&d.cell_cnt[cell_of(R, p)], 1u);
+      d.key[i] = cell_of(R, p);
+    }
   }
-  d.chainpos[4 * (size_t)new_amp + u] = make_double4(np[0], np[1], np[2], 0.0);
-  Saru sv = saru_seed(i, time, R.chem_seed ^ TAG_VEL);
-  for (int a = 0; a < 3; ++a) {
-    const double u1 = saru_uni_f64(sv);
-    const double u2 = saru_uni_f64(sv);
-    d.vel[a * cap + i] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-    d.vbuf[a * cap + i] = 0.0;
-    d.acc[a * cap + i] = 0.0f;
-    d.press[a * cap + i] = 0.0f;
-    d.delta[a * cap + i] = 0.0;
+}
+
+__global__ void k_chem_commit(DevState d, const ChemDev* __restrict__ cd,
+                              ChemState* __restrict__ cs, uint32_t* __restrict__ amp_num) {
+  if (halted(d)) return;
+  const uint32_t nt = cs->nt;
+  if (threadIdx.x == 0) {
+    if (nt) {
+      d.rep_off[1] = cs->n0 + 4u * nt;
+      d.amp_off[1] = cs->namp0 + nt;
+      amp_num[0] += nt;
+    }
+    cs->events += 1u;
+    cs->total += nt;
+    if (cs->refused) atomicOr(d.flags, FLAG_CHEM_CAPACITY);
+    // main.cpp:188: the run ends after the step whose event reached
+    // max_amp_num
+    if (cd->in_run && d.run_ok && (cs->stop || cs->refused)) *d.run_ok = 0u;
   }
-  d.id[i] = i;
-  d.orig[i] = i;
-  d.prop[i] = (u == 0) ? 0u : 1u;
-  d.amp_id[i] = new_amp;
-  d.unit[i] = u;
-  d.topo[4 * (size_t)new_amp + u] = i;
-  d.chain[i] = (new_amp << 2) | u;
-  if (u == 0) d.amp_rep[new_amp] = 0;
+  for (uint32_t c = threadIdx.x; c < cd->n_core; c += blockDim.x) cs->core_slot[c] = 0xffffffffu;
 }
-void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget, double eps,
-                        double bond_leng, uint32_t time, hipStream_t st) {
-  if (!ntarget) return;
-  hipLaunchKernelGGL(k_chem_insert, grid1(4 * ntarget, 64), dim3(64), 0, st, d, targets, ntarget,
-                     eps, bond_leng, time);
+
+void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
+                       uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
+                       hipStream_t st) {
+  const uint32_t nblk = div_up(d.cap / 4u + 1u, (uint32_t)kChemBlock);
+  if (n_core) {
+    hipLaunchKernelGGL(k_chem_find, grid1(d.nl, 256), dim3(256), 0, st, d, cd, cs);
+    hipLaunchKernelGGL(k_chem_core, dim3(div_up(n_core, 256)), dim3(256), 0, st, d, cd, cs);
+    hipLaunchKernelGGL(k_chem_count, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
+                       (const ChemState*)cs, counts);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, counts, 1u, 0u);
+  }
+  const uint32_t m = n_core ? n_core * nblk : 1u;
+  hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, counts, m);
+  if (n_core)
+    hipLaunchKernelGGL(k_chem_scatter, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
+                       (const ChemState*)cs, (const uint32_t*)counts, targets);
+  hipLaunchKernelGGL(k_chem_insert, dim3(256), dim3(256), 0, st, d, cd, (const ChemState*)cs,
+                     (const uint32_t*)targets, keys ? 1 : 0);
+  hipLaunchKernelGGL(k_chem_commit, dim3(1), dim3(256), 0, st, d, cd, cs, amp_num);
 }
+uint32_t chem_count_blocks(uint32_t cap) { return div_up(cap / 4u + 1u, (uint32_t)kChemBlock); }
 
 __global__ void k_copy_d(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1059,7 +1250,7 @@ void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st) {
 __global__ void __launch_bounds__(256) k_xyz_gather(DevState d, const double* __restrict__ vel,
                                                     XyzRec* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  if (i >= dev_n(d)) return;
   XyzRec r;
   const size_t cap = d.cap;
 #pragma unroll
@@ -1076,8 +1267,8 @@ __global__ void __launch_bounds__(256) k_xyz_gather(DevState d, const double* __
   out[d.orig[i]] = r;
 }
 void launch_xyz_gather(const DevState& d, const double* vel, XyzRec* out, hipStream_t st) {
-  if (!d.n) return;
-  hipLaunchKernelGGL(k_xyz_gather, grid1(d.n, 256), dim3(256), 0, st, d, vel, out);
+  if (!d.nl) return;
+  hipLaunchKernelGGL(k_xyz_gather, grid1(d.nl, 256), dim3(256), 0, st, d, vel, out);
 }
 
 }  // namespace mdpd

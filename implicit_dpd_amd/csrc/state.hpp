@@ -35,7 +35,14 @@ struct SBead {
 struct DevState : Master {
   Master alt;       // the other buffer set, target of the next permutation
   uint32_t cap;
-  uint32_t n;       // beads in use
+  uint32_t n;       // beads in use, as the host last knew them
+  uint32_t nl;      // launch bound of per-bead kernels: n, or cap in grow mode
+  // grow mode (set once chemistry is used): the live bead and chain counts
+  // are rep_off[n_rep] / amp_off[n_rep] on the device, which chemistry
+  // advances inside captured steps, and *run_ok = 0 makes every remaining
+  // kernel of a run return at once (the chemistry stopped the run); null
+  // otherwise
+  uint32_t* run_ok;
   // cell list, rebuilt every step
   uint32_t* key;        // [cap] global cell of each bead
   uint32_t* cell_cnt;   // [ncell + 1]
@@ -81,6 +88,40 @@ struct DevState : Master {
   uint32_t* flags;
   double* red;          // reduction partials
   double* red_out;      // reduction results
+};
+
+// live bead count: the device's in grow mode, the by-value host count otherwise
+__device__ __forceinline__ uint32_t dev_n(const DevState& d) {
+  return d.run_ok ? d.rep_off[d.n_rep] : d.n;
+}
+__device__ __forceinline__ uint32_t dev_namp(const DevState& d) {
+  return d.run_ok ? d.amp_off[d.n_rep] : d.n_amp;
+}
+// a chemistry stop ended the run: the step's kernels do nothing
+__device__ __forceinline__ bool halted(const DevState& d) {
+  return d.run_ok && *d.run_ok == 0u;
+}
+
+// RandomChemEvent parameters on the device (chemmanager.hpp:244-281 and the
+// Parameter fields it reads); slot 0 serves the chemistry inside mdpd_run,
+// slot 1 the explicit mdpd_chem_event.
+constexpr uint32_t kMaxCore = 256;
+struct ChemDev {
+  double p_thresld, eps, influ_rad, influ_hei, influ_dep, influ_grd, bond_leng, cf_b;
+  uint32_t max_amp_num, n_core, in_run, pad;
+  uint32_t core_id[kMaxCore];  // bead ids of the core heads (Parameter::core_ptcl_id)
+};
+// per-event state and results of a chemistry slot
+struct ChemState {
+  uint32_t nt;       // amphiphiles inserted by the last event
+  uint32_t n0, namp0;  // counts before it
+  uint32_t stop;     // n_amp >= max_amp_num after it
+  uint32_t refused;  // it would have exceeded the capacity (nothing inserted)
+  uint32_t events;   // events so far
+  uint32_t total;    // amphiphiles inserted so far
+  uint32_t pad;
+  uint32_t core_slot[kMaxCore];   // storage slot of each core head (~0: absent)
+  double core_geo[kMaxCore][6];   // corrected head position, membrane normal h2t
 };
 
 // launch wrappers (defined in the .hip files)
@@ -152,15 +193,6 @@ void launch_pairs_list(const DevState& d, double cutoff, uint32_t* out_i, uint32
                        unsigned long long* counter, uint64_t cap, hipStream_t st);
 void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3, uint32_t n,
                       uint32_t* u32out, float* nrml_out, hipStream_t st);
-// chemistry
-void launch_find_ids(const DevState& d, const uint32_t* ids, uint32_t nid, uint32_t* idx_out,
-                     hipStream_t st);
-void launch_chem_flags(const DevState& d, const double* core /*[n_core][6]: pos h, h2t*/,
-                       uint32_t n_core, double p_thresld, double influ_rad, double influ_hei,
-                       double influ_dep, double influ_grd, uint32_t time, uint8_t* flags,
-                       hipStream_t st);
-void launch_chem_insert(const DevState& d, const uint32_t* targets, uint32_t ntarget,
-                        double eps, double bond_leng, uint32_t time, hipStream_t st);
 void launch_copy_d(double* dst, const double* src, size_t n, hipStream_t st);
 // one bead's columns of the xyz trajectory format, in upload order
 struct XyzRec {

@@ -189,6 +189,21 @@ class Engine:
         self._c(self.L.mdpd_chem_event(self.h, C.byref(cp), C.byref(stop)))
         return stop.value
 
+    def set_chemistry(self, p_thresld, eps, influ_rad, influ_hei, influ_dep, influ_grd,
+                      max_amp_num, core_ptcl_id, beg_chem=0, enable=True):
+        """RandomChemEvent inside mdpd_run for steps with time >= beg_chem."""
+        ids = _u32(core_ptcl_id)
+        self._chem_ids = ids
+        cp = MdpdChemParams(p_thresld, eps, influ_rad, influ_hei, influ_dep, influ_grd,
+                            max_amp_num, len(ids), _u(ids) if len(ids) else None)
+        self._c(self.L.mdpd_set_chemistry(self.h, C.byref(cp), int(beg_chem), 1 if enable else 0))
+
+    def chem_status(self):
+        """(stopped, in-run events, amphiphiles inserted in runs)"""
+        st, ev, ins = C.c_int32(), C.c_uint32(), C.c_uint32()
+        self._c(self.L.mdpd_chem_status(self.h, C.byref(st), C.byref(ev), C.byref(ins)))
+        return bool(st.value), ev.value, ins.value
+
     def run(self, nsteps):
         self._c(self.L.mdpd_run(self.h, int(nsteps)))
 

@@ -205,8 +205,29 @@ int mdpd_remove_cm_drift(mdpd_ctx* ctx);        /* driftkick.hpp:42-54 */
 int mdpd_head_conversion(mdpd_ctx* ctx);        /* keyed type flip, no counterpart */
 /* RandomChemEvent (chemmanager.hpp:244-281) with keyed draws and the new
    amphiphiles appended to the bonded topology; *stop = 1 once
-   amp_num >= max_amp_num. */
+   amp_num >= max_amp_num.  The draws are keyed on the context clock.  The
+   event runs on the device with the context's own scratch (no allocation,
+   no copy of the topology); inserted beads do not invalidate the captured
+   steps of mdpd_run.  More amphiphiles than the capacity holds: nothing is
+   inserted and MDPD_ERR_CAPACITY. */
 int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
+/* The chemistry branch of the main loop (main.cpp:186-196) inside
+   mdpd_run: with enable != 0 every step whose time >= beg_chem ends with
+   RandomChemEvent after the CM-drift removal, at that step's time, inside
+   the captured step (the reference's beg_chem is a compile-time constant,
+   parameter.hpp:254; here a runtime field).  When an event brings amp_num to
+   max_amp_num the run ends after that step, as the reference's loop breaks
+   (main.cpp:188): the clock stays at that step's time, mdpd_chem_status
+   reports stopped = 1 and mdpd_run refuses to continue until
+   mdpd_set_chemistry is called again.  mdpd_run then synchronises at its end
+   (it reads back the bead count).  Single-system contexts with
+   verlet_skin = 0; at most 256 core amphiphiles.  enable = 0 switches it off. */
+int mdpd_set_chemistry(mdpd_ctx* ctx, const mdpd_chem_params* cp, uint32_t beg_chem,
+                       int32_t enable);
+/* stopped: an event inside the last mdpd_run reached max_amp_num; events and
+   inserted (may be NULL): in-run events and amphiphiles added since
+   mdpd_set_chemistry.  No reference counterpart. */
+int mdpd_chem_status(mdpd_ctx* ctx, int32_t* stopped, uint32_t* events, uint32_t* inserted);
 int mdpd_observe(mdpd_ctx* ctx, mdpd_observables* out);
 /* observables of every replica of an ensemble context, out[n_rep] */
 int mdpd_observe_replicas(mdpd_ctx* ctx, mdpd_observables* out);
