@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
 """Headline benchmark: MDPD bead-steps/s at 1M beads on one GPU.
 
-Workload: the implicit-solvent amphiphile membrane system, 262,144 H1T3 chains
+Workload: the implicit-solvent amphiphile membrane model, 262,144 H1T3 chains
 (1,048,576 beads) in a 100x100x40 box, Warren MDPD with A_ij in [-40,-20] by
 bead type, B = 25, rd = 0.75, harmonic bonds k = 100 l0 = 0.5, bending 10,
-head conversion p = 1e-4 per step, dt = 0.005 (synthetic seeded random
-chains).  One step is the reference's main-loop body (src/main.cpp:167-196):
-drift, cell-list rebuild, density pass, force pass, bonded forces, kick,
-centre-of-mass drift removal every 100 steps, head conversion, clock.
+head conversion p = 1e-4 per step, dt = 0.005.  The state is a membrane: five
+stacked bilayers (config_maker's flat sheet, src/config_maker.cpp:103-137,
+repeated along z), so the beads sit at the MDPD liquid density inside the
+bilayers.  `--workload membrane_gas` times the same model on seeded random
+chains spread over the box instead.  One step is the reference's main-loop
+body (src/main.cpp:167-196): drift, cell-list rebuild, density pass, force
+pass, bonded forces, kick, centre-of-mass drift removal every 100 steps, head
+conversion, clock.
 
 The timed region runs K steps as replayed hipGraphs (the production path)
 between two stream synchronisations; `value` is beads * K / (wall time of the
 region), and HIP events on the engine stream give its device time.  K more
 steps launched kernel by kernel with HIP events around every phase give the
 per-kernel times for the roofline of the dominant pair pass (density or
-force, whichever takes longer); its HBM traffic and VALU counters come from
-the committed rocprofv3 summary of this command.  The CPU
-baseline is the FP64 oracle (the reference's per-bead gather loops, OpenMP)
-timed on a bounded sample of the same workload.
+force, whichever takes longer); its HBM traffic, VALU counters and the
+rocprofv3 kernel-trace average of the same command come from the committed
+summaries in profiles/.  Then K steps with RandomChemEvent inside the
+captured steps (chemistry on) are timed the same way.  The CPU baseline is
+the FP64 oracle (the reference's per-bead gather loops, OpenMP) timed on a
+bounded sample of the same workload with every host thread and with one.
 """
 import argparse
 import json
@@ -26,12 +32,14 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "MDPD bead-steps/s at 1M beads, % HBM roofline; surrogate vs plain calib runs"
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "membrane_1M"
+CHEM_ROOM = 4 * 8192   # beads of headroom for the chemistry-on steps
 
 
 def log(*a):
@@ -54,17 +62,35 @@ def skin_density_bytes_per_launch(n, n_pairs, nd, n_skin):
     return n * (16 + 4 + 4 * nd + 4) + 4.0 * n_skin + 4.0 * n_pairs
 
 
-def force_bytes_per_launch(n, n_pairs, nd):
-    """Algorithmic HBM bytes of one force pass: per bead its own sorted
-    position (16) and velocity (16), densities (4 nd), pair count (4), slot
-    map (4), the acceleration written (12); per pair-list entry 4 bytes.
-    Neighbour data reuse is on-chip and not counted."""
-    return n * (16 + 16 + 4 * nd + 4 + 4 + 12) + 4.0 * n_pairs
+def force_bytes_per_launch(n, n_pairs, nd, n_chain_beads, n_heads, ncell, keys=True, conv=True):
+    """Algorithmic HBM bytes of one force pass with its fused epilogue (the
+    production step's variant).  Pair part: per bead its own 32-byte record
+    (position + predicted velocity, the density in its spare lane when
+    nd = 1, else 4 nd bytes of densities) and pair count (4); per pair-list
+    entry 4 bytes (neighbour records are reused on chip).  Epilogue: the
+    chain map (4 per bead) and the chain's FP64 positions (32 per chain
+    bead, each read once); the kick's vel_buf read (24), velocity (24) and
+    FP32 acceleration (12) written; with keys the master position read (24),
+    the cell key written (4) and the cell histogram (4 read + 4 written per
+    cell); with the head conversion every bead's unit (4) and the heads'
+    type and chain id (8)."""
+    b = n * (32 + 4 + (4 * nd if nd > 1 else 0)) + 4.0 * n_pairs
+    b += 4 * n + 32 * n_chain_beads
+    b += (24 + 24 + 12) * n
+    if keys:
+        b += (24 + 4) * n + 8.0 * ncell
+    if conv:
+        b += 4 * n + 8 * n_heads
+    return b
+
+
+VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9   # CUs x lanes x clock: non-packed lane-ops/s
 
 
 def profiled_counters(workload):
-    """Per-launch HBM bytes and VALU counters of each kernel from the
-    committed rocprofv3 summary of this workload (profiles/), if present."""
+    """Per-launch HBM bytes, VALU counters and kernel-trace durations of each
+    kernel from the committed rocprofv3 summary of this workload (profiles/),
+    if present."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.txt")
     out = {}
     if not os.path.exists(path):
@@ -76,7 +102,8 @@ def profiled_counters(workload):
             kv = dict(tok.split("=", 1) for tok in line.split())
             k = kv.get("variant", kv.get("kernel"))
             out[k] = {"traffic_bytes": float(kv["traffic_bytes"]), "avg_ns": float(kv["avg_ns"]),
-                      "valu_busy": float(kv["valu_busy"]), "valu_util": float(kv["valu_util"])}
+                      "valu_busy": float(kv["valu_busy"]), "valu_util": float(kv["valu_util"]),
+                      "valu_insts": float(kv.get("valu_insts", "nan"))}
     return out
 
 
@@ -97,9 +124,15 @@ def step_bytes_per_bead(nd, bonded, rho_mean):
     return 264 + 8 * nd + (16 if bonded else 0) + 16.0 / rho_mean
 
 
-def cpu_baseline(p, s, topo, budget_s, max_steps):
+def cpu_baseline(p, s, topo, budget_s, max_steps, threads=None):
+    """The FP64 oracle on the same system: steps until budget_s has passed
+    (at most max_steps) after one warm-up step, with `threads` OpenMP threads
+    (all host threads when None)."""
     from oracle import pyoracle as po
     n = len(s["pos"])
+    nthr0 = po.num_threads()
+    if threads:
+        po.set_num_threads(threads)
     o = po.System(p, n, max(len(topo) // 4, 1))
     o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
     if len(topo):
@@ -114,10 +147,21 @@ def cpu_baseline(p, s, topo, budget_s, max_steps):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "bead-steps/s", "cores": po.num_threads(),
-            "kind": "port",
+    cores = po.num_threads()
+    po.set_num_threads(nthr0)
+    return {"value": n * steps / dt, "unit": "bead-steps/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle (per-bead gather loops, OpenMP) on the same {n}-bead "
                       f"system, {steps} steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def calibration_record():
+    """The committed record of the surrogate-guided calibration against plain
+    sweeps (the metric's third clause), made by implicit_dpd_amd.calibrate."""
+    path = os.path.join(ROOT, "profiles", "calibration_runs.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 class Workload:
@@ -141,8 +185,9 @@ class Workload:
         the warm-up steps."""
         from implicit_dpd_amd import configs
         if e is None:
+            # room for the amphiphiles the chemistry-on steps insert
             e = (Engine(self.params, n_beads=self.sizes) if self.ensemble
-                 else Engine(self.params, capacity=self.n))
+                 else Engine(self.params, capacity=self.n + (CHEM_ROOM if self.bonded else 0)))
             if self.ensemble:
                 e.set_tuning(nlist_width=256)  # dense corners of the sampled range
             if skin > 0.0:
@@ -159,11 +204,16 @@ class Workload:
 
 def workload(name, chains):
     from implicit_dpd_amd import configs
+    model = ("Warren MDPD A_ij in [-40,-20] by type, B=25, rd=0.75, bonds k=100 l0=0.5, bending "
+             "10, head conversion p=1e-4/step, dt=0.005")
     if name == "membrane":
+        p, s = configs.membrane_stack(n_chain=chains)
+        return Workload(name, p, s, f"membrane: {chains} H1T3 chains ({4 * chains} beads) as 5 "
+                        f"stacked bilayers, box 100x100x40, {model}", True)
+    if name == "membrane_gas":
         p, s = configs.membrane(n_chain=chains)
-        return Workload(name, p, s, f"membrane: {chains} H1T3 chains ({4 * chains} beads), box "
-                        "100x100x40, Warren MDPD A_ij in [-40,-20] by type, B=25, rd=0.75, bonds "
-                        "k=100 l0=0.5, bending 10, head conversion p=1e-4/step, dt=0.005", True)
+        return Workload(name, p, s, f"membrane model on random chains: {chains} H1T3 chains "
+                        f"({4 * chains} beads) spread over the box 100x100x40, {model}", True)
     if name == "slab":
         p, s = configs.mdpd_slab()
         return Workload(name, p, s, "MDPD slab: 198246 beads in the middle half of a 32x32x64 box, "
@@ -299,10 +349,12 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="membrane",
-                    choices=["membrane", "slab", "groot_warren", "ensemble"],
-                    help="membrane is the headline system; the others are the smaller "
+                    choices=["membrane", "membrane_gas", "slab", "groot_warren", "ensemble"],
+                    help="membrane (stacked bilayers) is the headline system; membrane_gas the "
+                         "same model on random chains; the others are the smaller "
                          "configurations of the parity tests")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-chem", action="store_true", help="skip the chemistry-on steps")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--chains", type=int, default=262144)
     ap.add_argument("--autotune", action="store_true")
@@ -388,22 +440,36 @@ def main():
     n_pairs = obs["n_pairs"]
     nd = w.nd
 
-    prof = profiled_counters(WORKLOAD if w.name == "membrane" else f"{w.name}")
+    prof = profiled_counters(f"{w.name}_1M" if w.name.startswith("membrane") else w.name)
+    st0 = w.systems if not w.ensemble else None
+    n_chain_beads = int((st0["unit"] != 0xFFFFFFFF).sum()) if w.bonded else 0
+    n_heads = int((st0["unit"] == 0).sum()) if w.bonded else 0
     kern = {}
     dens_algo = (skin_density_bytes_per_launch(n, n_pairs, nd, n_skin) if args.skin > 0
                  else density_bytes_per_launch(n, n_pairs, nd, w.ncell))
+    force_algo = force_bytes_per_launch(n, n_pairs, nd, n_chain_beads, n_heads, w.ncell,
+                                        keys=args.skin == 0, conv=w.params.conv_p > 0
+                                        if not w.ensemble else False)
     for name, phase, algo in (
             ("k_density_skin" if args.skin > 0 else "k_density_q", "density", dens_algo),
-            ("k_force", "force", force_bytes_per_launch(n, n_pairs, nd))):
+            ("k_force", "force", force_algo)):
         ms = ph[phase]
         gbs = algo / (ms * 1e-3) / 1e9
         variant, pc = pick(prof, name)
-        kern[name] = {"ms": round(ms, 5), "algorithmic_bytes": int(algo), "achieved_gbs": round(gbs, 2),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4),
-                      "traffic": int(pc["traffic_bytes"]) if pc else None,
-                      "valu_busy_pct": pc["valu_busy"] if pc else None,
-                      "valu_lane_util_pct": pc["valu_util"] if pc else None,
-                      "profiled_variant": variant}
+        k = {"ms": round(ms, 5), "algorithmic_bytes": int(algo), "achieved_gbs": round(gbs, 2),
+             "frac": round(gbs / HBM_PEAK_GBS, 4), "profiled_variant": variant}
+        if pc:
+            rms = pc["avg_ns"] * 1e-6
+            lane_ops = pc["valu_insts"] * 64 * pc["valu_util"] / 100.0
+            k.update({"ms_rocprof": round(rms, 5),
+                      "frac_rocprof": round(algo / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": int(pc["traffic_bytes"]),
+                      "traffic_over_algorithmic": round(pc["traffic_bytes"] / algo, 2),
+                      "valu_busy_pct": pc["valu_busy"], "valu_lane_util_pct": pc["valu_util"],
+                      "valu_lane_ops": lane_ops if math.isfinite(lane_ops) else None,
+                      "valu_frac": (round(lane_ops / (VALU_LANE_OPS_PEAK * rms * 1e-3), 4)
+                                    if math.isfinite(lane_ops) else None)})
+        kern[name] = k
     dom = max(kern, key=lambda k: kern[k]["ms"])
     kd = kern[dom]
     bstep = step_bytes_per_bead(nd, w.bonded, rho_mean)
@@ -413,27 +479,71 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kd["frac"],
-        "traffic": kd["traffic"],
+        "traffic": kd.get("traffic"),
         "kernel": dom,
         "kernel_ms": kd["ms"],
+        "kernel_ms_source": "HIP events around the kernel in an eager replay of the timed steps "
+                            "(the same kernels the graphs launch); ms_rocprof: rocprofv3 "
+                            "kernel-trace average of this command (profiles/)",
         "algorithmic_bytes_per_launch": kd["algorithmic_bytes"],
         "pairs_per_bead": round(n_pairs / n, 3),
-        "binding_roof": "VALU (FP32/INT32 lane ops: candidate tests, Saru hash, log/sqrt/cos); "
-                        "valu_busy_pct / valu_lane_util_pct from the committed rocprofv3 counters",
-        "valu_busy_pct": kd["valu_busy_pct"],
-        "valu_lane_util_pct": kd["valu_lane_util_pct"],
+        "binding_roof": "VALU (FP32/INT32 lane ops: candidate tests, Saru hash, log/sqrt/cos): "
+                        "valu_frac = SQ_INSTS_VALU x 64 x VALUUtilization / (256 CUs x 64 lanes x "
+                        "2.4 GHz x kernel time), from the committed rocprofv3 counters",
+        "valu_frac": kd.get("valu_frac"),
+        "valu_busy_pct": kd.get("valu_busy_pct"),
+        "valu_lane_util_pct": kd.get("valu_lane_util_pct"),
         "pair_kernels": kern,
     }
     step_hbm = {"bytes_per_bead_step": round(bstep, 1),
                 "achieved_gbs": round(value * bstep / 1e9, 1),
                 "frac": round(value * bstep / (HBM_PEAK_GBS * 1e9), 4)}
 
+    chem = None
+    if w.bonded and not w.ensemble and args.skin == 0 and not args.no_chem:
+        # RandomChemEvent inside the captured steps from the same start: four
+        # core amphiphiles, cones of radius 3, depth -1..3 around them
+        w.start(Engine, e=e, warmup=args.warmup)
+        core = np.array([4 * a for a in (0, 1, 2, 3)], np.uint32)
+        n_amp0 = e.observe()["n_amp"]
+        e.set_chemistry(p_thresld=2e-3, eps=0.2, influ_rad=3.0, influ_hei=1.0, influ_dep=3.0,
+                        influ_grd=0.0, max_amp_num=10 ** 9, core_ptcl_id=core, beg_chem=0)
+        e.run(3)  # captures the chemistry steps' graphs
+        e.sync()
+        t = time.perf_counter()
+        e.run(args.steps)
+        cwall = time.perf_counter() - t
+        stopped, events, inserted = e.chem_status()
+        # a few more chemistry steps launched kernel by kernel: the event's
+        # own time (the "reactions" phase)
+        e.set_profiling(True)
+        e.run(20)
+        e.sync()
+        cph = e.phase_times()
+        e.set_profiling(False)
+        e.set_chemistry(0, 0, 0, 0, 0, 0, 0, [], enable=False)
+        chem = {"ms_per_step": round(cwall * 1e3 / args.steps, 5),
+                "ms_per_step_without": round(ms_step, 5),
+                "event_ms_eager": round(cph.get("reactions", float("nan")), 5),
+                "events": events, "amphiphiles_inserted": inserted,
+                "beads_end": e.num_beads(), "n_amp_start": n_amp0,
+                "setup": "RandomChemEvent (cone target test, keyed draws) at the end of every "
+                         "step inside the captured steps: cores = amphiphiles 0-3, p_thresld 2e-3, "
+                         "influ_rad 3, influ_hei 1, influ_dep 3, influ_grd 0, eps 0.2; mdpd_run "
+                         "synchronises once at its end to read back the bead count"}
+        log(f"[bench] chemistry on: {chem['ms_per_step']:.4f} ms/step, {inserted} amphiphiles "
+            f"inserted in {events} events")
+
     cpu = None
     if not args.no_cpu and not w.ensemble:
         log("[bench] CPU baseline (oracle)...")
         s = w.systems
-        cpu = cpu_baseline(w.params, s, s["topo"] if w.bonded else [], args.cpu_seconds, 400)
+        topo = s["topo"] if w.bonded else []
+        cpu = cpu_baseline(w.params, s, topo, args.cpu_seconds, 400)
         log(f"[bench] CPU: {cpu['value']:.4g} bead-steps/s on {cpu['cores']} threads")
+        one = cpu_baseline(w.params, s, topo, args.cpu_seconds / 2, 400, threads=1)
+        cpu["one_thread"] = {k: one[k] for k in ("value", "cores", "sample")}
+        log(f"[bench] CPU: {one['value']:.4g} bead-steps/s on 1 thread")
 
     out = {
         "metric": METRIC,
@@ -461,7 +571,9 @@ def main():
         },
         "roofline": roof,
         "step_hbm_roofline": step_hbm,
+        "chemistry_on": chem,
         "cpu_baseline": cpu,
+        "calibration": calibration_record(),
     }
     print(json.dumps(out), flush=True)
     e.close()

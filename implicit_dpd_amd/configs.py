@@ -218,6 +218,44 @@ def membrane(seed=3, n_chain=262144, box=(100.0, 100.0, 40.0), conv_p=1e-4):
     return p, random_chains(n_chain, box, seed)
 
 
+def lamellar_chains(n_chain, box, seed, n_layers=5, bond=0.5, jitter=0.05):
+    """H1T3 amphiphiles as a stack of n_layers flat bilayers along z, each
+    the flat sheet of config_maker (src/config_maker.cpp:103-137): in-plane
+    positions uniform, leaflets alternating, the head at 3 bond + 1e-5 from
+    the sheet's mid-plane and the chain straight towards it, here with a
+    small Gaussian jitter on every bead; layer k is centred at
+    (k + 1/2) Lz / n_layers.  Amphiphile a occupies beads 4a..4a+3."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=np.float64)
+    layer = np.arange(n_chain) % n_layers
+    zc = (layer + 0.5) * box[2] / n_layers
+    sign = np.where((np.arange(n_chain) // n_layers) % 2 == 0, 1.0, -1.0)
+    head = np.column_stack([rng.random(n_chain) * box[0], rng.random(n_chain) * box[1],
+                            zc + (3 * bond + 1e-5) * sign])
+    u = np.arange(4, dtype=np.float64)
+    pos = head[:, None, :].repeat(4, axis=1)
+    pos[:, :, 2] -= sign[:, None] * bond * u[None, :]
+    pos += rng.normal(scale=jitter, size=pos.shape)
+    pos = pos.reshape(-1, 3)
+    pos -= np.floor(pos / box) * box
+    pos = np.minimum(pos, np.nextafter(box, 0))
+    n = 4 * n_chain
+    return dict(pos=pos, vel=mb_velocities(rng, n), ids=np.arange(n, dtype=np.uint32),
+                prop=np.tile(np.array([HYPHIL, HYPHOB, HYPHOB, HYPHOB], np.uint32), n_chain),
+                amp_id=np.repeat(np.arange(n_chain, dtype=np.uint32), 4),
+                unit=np.tile(np.arange(4, dtype=np.uint32), n_chain),
+                topo=np.arange(n, dtype=np.uint32))
+
+
+def membrane_stack(seed=5, n_chain=262144, box=(100.0, 100.0, 40.0), n_layers=5, conv_p=1e-4):
+    """The membrane model of membrane() on an actual membrane state: the
+    262,144 H1T3 chains as n_layers stacked bilayers (lamellar phase) of
+    ~52k amphiphiles each, so the beads sit at the MDPD liquid density
+    (~6 per rc^3) inside the bilayers instead of a uniform 2.6."""
+    p, _ = membrane(seed=seed, n_chain=1, box=box, conv_p=conv_p)
+    return p, lamellar_chains(n_chain, box, seed, n_layers=n_layers)
+
+
 def ensemble(seed=4, n_rep=64, nmin=4096, nmax=65536, rho=6.0):
     """Replicas of uneven size (U[nmin, nmax] beads), A ~ U[-60,-20],
     B ~ U[10,40], rd ~ U[0.65,0.85], kBT = 1, independent seeds; each a

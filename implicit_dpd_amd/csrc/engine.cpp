@@ -17,7 +17,7 @@ using namespace mdpd;
 namespace mdpd {
 thread_local std::string g_err;
 const char* kPhaseNames[kPhases] = {"drift", "sort", "gather", "density", "force",
-                                    "bonded_kick", "kick", "cm_drift", "conversion", "clock"};
+                                    "bonded_kick", "kick", "cm_drift", "reactions", "clock"};
 
 int fail(mdpd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;

@@ -988,23 +988,79 @@ __global__ void __launch_bounds__(256) k_chem_find(DevState d, const ChemDev* __
     if (cd->core_id[c] == id) cs->core_slot[c] = s;
 }
 
-__global__ void __launch_bounds__(256) k_chem_core(DevState d, const ChemDev* __restrict__ cd,
-                                                   ChemState* __restrict__ cs) {
+// one block of kCoreBlock threads per core: the neighbour sums of the core
+// head spread over the threads (cells of its window, or beads on the
+// all-pairs path), then a fixed-order tree reduction, so the result does not
+// depend on scheduling
+constexpr int kCoreBlock = 128;
+__global__ void __launch_bounds__(kCoreBlock) k_chem_core(DevState d, const ChemDev* __restrict__ cd,
+                                                          ChemState* __restrict__ cs) {
 #pragma clang fp contract(off)
   if (halted(d)) return;
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = blockIdx.x;
   if (c >= cd->n_core) return;
+  __shared__ double red[8][kCoreBlock];
   const Replica& R = d.reps[0];
+  const uint32_t slot = cs->core_slot[c];
+  const int tid = threadIdx.x;
+  double sl[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0}, nl = 0.0, nb = 0.0;
+  if (slot != 0xffffffffu) {
+    double ri[3];
+    for (int a = 0; a < 3; ++a) ri[a] = d.pos[a * d.cap + slot];
+    auto visit = [&](uint32_t j) {
+      if (j == slot) return;
+      double dv[3];
+      const double r2 = fp64_disp(d, R, slot, j, dv);
+      if (r2 == 0.0) return;
+      const uint32_t pj = d.prop[j];
+      if (r2 < 1.44 && pj == 1u) { for (int a = 0; a < 3; ++a) sb[a] += ri[a] - dv[a]; nb += 1.0; }
+      if (r2 < 1.0 && pj == 0u) { for (int a = 0; a < 3; ++a) sl[a] += ri[a] - dv[a]; nl += 1.0; }
+    };
+    int rad[3];
+    bool brute = false;
+    for (int a = 0; a < 3; ++a) {
+      rad[a] = (int)ceil((1.2 + d.cell_slack) * R.inv_h[a] - 1e-12);
+      if (2 * rad[a] + 1 > R.nc[a]) brute = true;
+    }
+    if (brute) {
+      const uint32_t b1 = d.rep_off[1];
+      for (uint32_t j = (uint32_t)tid; j < b1; j += kCoreBlock) visit(j);
+    } else {
+      const uint32_t cc = d.scell[slot] - R.cell_off;
+      const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+      const int cx = (int)(cc % (uint32_t)nx), cy = (int)((cc / (uint32_t)nx) % (uint32_t)ny),
+                cz = (int)(cc / (uint32_t)(nx * ny));
+      const int wx = 2 * rad[0] + 1, wy = 2 * rad[1] + 1, wz = 2 * rad[2] + 1;
+      for (int w = tid; w < wx * wy * wz; w += kCoreBlock) {
+        const int ox = w % wx - rad[0], oy = (w / wx) % wy - rad[1], oz = w / (wx * wy) - rad[2];
+        const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
+        const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
+        for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) visit(t);
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[a][tid] = sl[a];
+    red[3 + a][tid] = sb[a];
+  }
+  red[6][tid] = nl;
+  red[7][tid] = nb;
+  __syncthreads();
+  for (int h = kCoreBlock / 2; h > 0; h >>= 1) {
+    if (tid < h)
+      for (int q = 0; q < 8; ++q) red[q][tid] += red[q][tid + h];
+    __syncthreads();
+  }
+  if (tid != 0) return;
   // GetCorePos: (-1, -1, -1) when the core head is absent
   double h[3] = {-1.0, -1.0, -1.0}, t[3] = {-1.0, -1.0, -1.0}, m[3] = {-1.0, -1.0, -1.0};
-  const uint32_t slot = cs->core_slot[c];
   if (slot != 0xffffffffu) {
-    double o[8];
-    neighbor_cm_at(d, slot, o);
+    // the hydrophilic sum starts with the head itself (count 1)
+    const double cl = 1.0 + red[6][0], cb = red[7][0];
     for (int a = 0; a < 3; ++a) {
       h[a] = d.pos[a * d.cap + slot];
-      m[a] = o[a];
-      t[a] = o[3 + a];
+      m[a] = (h[a] + red[a][0]) / cl;
+      t[a] = red[3 + a][0] / cb;
     }
   }
   double h2t[3] = {t[0] - h[0], t[1] - h[1], t[2] - h[2]};
@@ -1219,7 +1275,7 @@ void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint
   const uint32_t nblk = div_up(d.cap / 4u + 1u, (uint32_t)kChemBlock);
   if (n_core) {
     hipLaunchKernelGGL(k_chem_find, grid1(d.nl, 256), dim3(256), 0, st, d, cd, cs);
-    hipLaunchKernelGGL(k_chem_core, dim3(div_up(n_core, 256)), dim3(256), 0, st, d, cd, cs);
+    hipLaunchKernelGGL(k_chem_core, dim3(n_core), dim3(kCoreBlock), 0, st, d, cd, cs);
     hipLaunchKernelGGL(k_chem_count, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
                        (const ChemState*)cs, counts);
   } else {

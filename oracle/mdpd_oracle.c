@@ -901,6 +901,14 @@ int oracle_num_threads(void) {
 #endif
 }
 
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ======================================================================
  * Initial configurations -- src/config_maker.cpp, walked sequentially as
  * the reference does: GenConfig (:342-384), MakeFlatSheet (:103-138, flap

@@ -128,6 +128,8 @@ void oracle_sys_observe(oracle_sys* s, uint32_t sol_num,
 /* nsteps of the main loop body; returns 0 or 1 if chemistry asked to stop */
 int oracle_sys_run(oracle_sys* s, uint32_t nsteps);
 int oracle_num_threads(void);
+/* OpenMP threads of the following calls (the CPU baseline's 1-thread figure) */
+void oracle_set_num_threads(int n);
 
 /* ---- initial configurations (src/config_maker.cpp) ---- */
 int oracle_config_count(const mdpd_config_spec* s, uint32_t* n);

@@ -61,6 +61,7 @@ _PROTOS = {
     "oracle_sys_observe": (None, [VP, C.c_uint32, P(MdpdObservables)]),
     "oracle_sys_run": (C.c_int, [VP, C.c_uint32]),
     "oracle_num_threads": (C.c_int, []),
+    "oracle_set_num_threads": (None, [C.c_int]),
     "oracle_config_count": (C.c_int, [P(MdpdConfigSpec), _U32P]),
     "oracle_make_config": (C.c_int, [P(MdpdConfigSpec), C.c_uint32, _DP, _DP, _U32P, _U32P, _U32P,
                                      _U32P, _DP]),
@@ -301,6 +302,10 @@ class System:
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n):
+    lib().oracle_set_num_threads(int(n))
 
 
 def make_config(spec):
