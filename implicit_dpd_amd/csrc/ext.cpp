@@ -71,16 +71,37 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
 }
 
 // ---- binary checkpoint (exact restart)
-// Layout, little-endian: magic "MDPDCKP1", u32 version, n_rep, n, n_amp,
-// time, 0; u32 rep_off[n_rep + 1], pair_seed[n_rep], amp_num[n_rep],
-// sol_num[n_rep]; FP64 pos, vel, vel_buf, delta_sumr [3][n]; FP32 acc,
-// press [3][n]; u32 id, prop, amp_id, unit, orig, chain [n]; u32
-// topo[4 n_amp]; FP64 cf_bend[n_amp]; u32 amp_rep[n_amp].  Arrays are in the
-// device's storage order, so the next cell sort (ordered by previous slot)
-// and every FP32 sum after it repeat the uninterrupted run.
+// Layout, little-endian: magic "MDPDCKP1", u32 version (2), n_rep, n, n_amp,
+// time, 0; u32 rep_off[n_rep + 1], pair_seed[n_rep], chem_seed[n_rep],
+// amp_num[n_rep], sol_num[n_rep]; u64 fingerprint[n_rep] of each replica's
+// parameters; FP64 pos, vel, vel_buf, delta_sumr [3][n]; FP32 acc, press
+// [3][n]; u32 id, prop, amp_id, unit, orig, chain [n]; u32 topo[4 n_amp];
+// FP64 cf_bend[n_amp]; u32 amp_rep[n_amp].  Arrays are in the device's
+// storage order, so the next cell sort (ordered by previous slot) and every
+// FP32 sum after it repeat the uninterrupted run.
 namespace {
 constexpr char kCkpMagic[8] = {'M', 'D', 'P', 'D', 'C', 'K', 'P', '1'};
-constexpr uint32_t kCkpVersion = 1;
+constexpr uint32_t kCkpVersion = 2;
+
+// FNV-1a over a replica's parameters with the seeds (restored from the
+// file) and the padding cleared: a checkpoint only loads into a context of
+// the same box and model
+uint64_t param_fingerprint(const mdpd_params& p0) {
+  mdpd_params p;
+  std::memset(&p, 0, sizeof(p));
+  std::memcpy(&p, &p0, sizeof(p));
+  p.pair_seed = 0;
+  p.chem_seed = 0;
+  p.pad0 = 0;
+  p.pad1 = 0;
+  uint64_t h = 1469598103934665603ull;
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(&p);
+  for (size_t k = 0; k < sizeof(p); ++k) {
+    h ^= b[k];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
 }  // namespace
 
 int mdpd_checkpoint_save(mdpd_ctx* c, const char* path) {
@@ -101,8 +122,13 @@ int mdpd_checkpoint_save(mdpd_ctx* c, const char* path) {
   put(head, sizeof(head));
   put(c->rep_off.data(), 4 * (size_t)(nr + 1));
   for (uint32_t r = 0; r < nr; ++r) put(&c->R[r].pair_seed, 4);
+  for (uint32_t r = 0; r < nr; ++r) put(&c->R[r].chem_seed, 4);
   put(c->amp_num_rep.data(), 4 * (size_t)nr);
   put(c->sol_num_rep.data(), 4 * (size_t)nr);
+  for (uint32_t r = 0; r < nr; ++r) {
+    const uint64_t fp = param_fingerprint(c->P[r]);
+    put(&fp, 8);
+  }
   int rc = MDPD_OK;
   auto get3d = [&](const double* src) -> int {
     for (int a = 0; a < 3; ++a)
@@ -162,19 +188,32 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
   get(head, sizeof(head));
   if (!ok || std::memcmp(magic, kCkpMagic, 8) != 0 || head[0] != kCkpVersion) {
     fclose(fp);
-    return fail(c, MDPD_ERR_PARSE, std::string("not a checkpoint of this format: ") + path);
+    return fail(c, MDPD_ERR_PARSE, std::string("not a checkpoint of this format (version 2): ") + path);
   }
   const uint32_t nr = head[1], n = head[2], na = head[3];
   if ((int)nr != c->n_rep || n > c->cap || na > c->amp_cap) {
     fclose(fp);
     return fail(c, MDPD_ERR_ARG, "checkpoint does not fit this context (replicas or capacity)");
   }
-  std::vector<uint32_t> roff(nr + 1), seed(nr), anum(nr), snum(nr);
+  std::vector<uint32_t> roff(nr + 1), seed(nr), cseed(nr), anum(nr), snum(nr);
+  std::vector<uint64_t> fprint(nr);
   get(roff.data(), 4 * (size_t)(nr + 1));
   get(seed.data(), 4 * (size_t)nr);
+  get(cseed.data(), 4 * (size_t)nr);
   get(anum.data(), 4 * (size_t)nr);
   get(snum.data(), 4 * (size_t)nr);
-  if (!ok || (nr > 1 && roff != c->rep_off)) {
+  get(fprint.data(), 8 * (size_t)nr);
+  if (!ok) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_PARSE, std::string("truncated checkpoint: ") + path);
+  }
+  for (uint32_t r = 0; r < nr; ++r)
+    if (fprint[r] != param_fingerprint(c->P[r])) {
+      fclose(fp);
+      return fail(c, MDPD_ERR_ARG, "checkpoint of other parameters (box or model) than replica " +
+                                       std::to_string(r) + " of this context");
+    }
+  if (nr > 1 ? roff != c->rep_off : (roff[0] != 0 || roff[1] != n)) {
     fclose(fp);
     return fail(c, MDPD_ERR_ARG, "checkpoint replica layout differs from this context");
   }
@@ -190,8 +229,57 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
     get(cf.data(), 8 * (size_t)na);
     get(u32.data() + 6 * (size_t)n + 4 * (size_t)na, 4 * (size_t)na);
   }
+  // nothing more may follow
+  char extra;
+  if (ok && fread(&extra, 1, 1, fp) == 1) ok = false;
   fclose(fp);
-  if (!ok) return fail(c, MDPD_ERR_PARSE, std::string("truncated checkpoint: ") + path);
+  if (!ok) return fail(c, MDPD_ERR_PARSE, std::string("truncated or oversized checkpoint: ") + path);
+  // every word that a kernel uses as an index is checked here, before
+  // anything reaches the device: a corrupt file is refused and the context
+  // keeps its state
+  {
+    const uint32_t* prop = u32.data() + 1 * (size_t)n;
+    const uint32_t* orig = u32.data() + 4 * (size_t)n;
+    const uint32_t* chain = u32.data() + 5 * (size_t)n;
+    const uint32_t* topo = u32.data() + 6 * (size_t)n;
+    const uint32_t* arep = u32.data() + 6 * (size_t)n + 4 * (size_t)na;
+    std::string why;
+    uint64_t asum = 0;
+    for (uint32_t r = 0; r < nr; ++r) asum += anum[r];
+    if (asum != na) why = "amphiphile counts do not add up";
+    for (uint32_t r = 0; r < nr && why.empty(); ++r)
+      if (snum[r] > roff[r + 1] - roff[r]) why = "solvent count exceeds replica " + std::to_string(r);
+    std::vector<uint8_t> seen_orig(n, 0), seen_chain(4 * (size_t)na, 0);
+    for (uint32_t r = 0; r < nr && why.empty(); ++r)
+      for (uint32_t i = roff[r]; i < roff[r + 1] && why.empty(); ++i) {
+        if (prop[i] > 2u) { why = "bead type > 2 at slot " + std::to_string(i); break; }
+        if (orig[i] >= n || seen_orig[orig[i]]++) { why = "slot map is not a permutation"; break; }
+        if (chain[i] != 0xffffffffu) {
+          if ((chain[i] >> 2) >= na || seen_chain[chain[i]]++) {
+            why = "chain word out of range or repeated at slot " + std::to_string(i);
+            break;
+          }
+        }
+        for (int a = 0; a < 3; ++a) {
+          const double x = f64[(size_t)a * n + i];
+          if (!std::isfinite(x) || x < 0.0 || x > c->P[r].box[a]) {
+            why = "position outside the box at slot " + std::to_string(i);
+            break;
+          }
+          for (int q = 1; q < 4; ++q)
+            if (!std::isfinite(f64[((size_t)q * 3 + a) * n + i])) why = "non-finite state at slot " + std::to_string(i);
+          for (int q = 0; q < 2; ++q)
+            if (!std::isfinite(f32[((size_t)q * 3 + a) * n + i])) why = "non-finite force at slot " + std::to_string(i);
+        }
+      }
+    for (size_t k = 0; k < 4 * (size_t)na && why.empty(); ++k) {
+      if (topo[k] >= n) why = "topology entry out of range";
+      else if (!seen_chain[k]) why = "chain slot without its bead";
+    }
+    for (uint32_t a = 0; a < na && why.empty(); ++a)
+      if (arep[a] >= nr || !std::isfinite(cf[a])) why = "chain replica or bending constant invalid";
+    if (!why.empty()) return fail(c, MDPD_ERR_PARSE, "corrupt checkpoint (" + why + "): " + path);
+  }
   DevState& d = c->d;
   double* const dd[4] = {d.pos, d.vel, d.vbuf, d.delta};
   for (int q = 0; q < 4; ++q)
@@ -220,6 +308,8 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
   for (uint32_t r = 0; r < nr; ++r) {
     c->R[r].pair_seed = seed[r];
     c->P[r].pair_seed = seed[r];
+    c->R[r].chem_seed = cseed[r];
+    c->P[r].chem_seed = cseed[r];
   }
   if (nr == 1) {
     c->rep_off[1] = n;

@@ -224,9 +224,11 @@ int mdpd_read_xyz_header(const char* path, uint32_t* n, uint32_t* time) {
   if (!path || !n || !time) return fail(nullptr, MDPD_ERR_ARG, "null argument");
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(nullptr, MDPD_ERR_PARSE, std::string("cannot open ") + path);
-  char buf[256];
-  const size_t k = std::fread(buf, 1, sizeof(buf), f);
+  // one spare byte: the parse below (strtoul) stops at a NUL at the latest
+  char buf[257];
+  const size_t k = std::fread(buf, 1, sizeof(buf) - 1, f);
   std::fclose(f);
+  buf[k] = '\0';
   const char* body = nullptr;
   std::string err;
   const int rc = xyz_header(buf, buf + k, n, time, &body, err);

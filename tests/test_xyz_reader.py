@@ -116,3 +116,29 @@ def test_capacity_and_missing_file(tmp_path):
     with pytest.raises(MdpdError) as e:
         read_xyz_native(tmp_path / "absent.xyz")
     assert e.value.code == capi.MDPD_ERR_PARSE
+
+
+@pytest.mark.parametrize("text,ok", [
+    (b"", False),
+    (b"   \n", False),
+    (b"7", False),                      # no time line
+    (b"7\n", False),
+    (b"7\ntime", False),
+    (b"7\ntime 12", True),              # header only, no trailing newline
+    (b"7\ntime 12\n", True),
+    (b"7\ntime " + b"9" * 300, False),  # longer than the header buffer
+    (b"7" * 300, False),
+])
+def test_header_only_and_malformed_headers(tmp_path, text, ok):
+    """mdpd_read_xyz_header reads at most the first 256 bytes and parses
+    them as a terminated string: empty, short and overlong headers are parse
+    errors, never reads past the buffer."""
+    import ctypes as C
+    path = tmp_path / "h.xyz"
+    path.write_bytes(text)
+    n, t = C.c_uint32(), C.c_uint32()
+    rc = capi.lib().mdpd_read_xyz_header(str(path).encode(), C.byref(n), C.byref(t))
+    if ok:
+        assert rc == 0 and (n.value, t.value) == (7, 12)
+    else:
+        assert rc == capi.MDPD_ERR_PARSE
