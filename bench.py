@@ -343,6 +343,68 @@ def traj_io_bench(args, Engine):
     e.close()
 
 
+def driver_bench(args, Engine):
+    """The drop-in program (implicit_dpd_amd/bin/mdpd_main, main.cpp's control
+    flow over the C++ facade) on the headline state: the lamellar membrane
+    written as init_config.xyz with a run_param.txt of the reference's own
+    multibody model (CalcInterCoef of chi 30, kappa 50, rho_co 25), all_time =
+    K + 1 steps, step_mac 100 (observations), no trajectory frames; three
+    bilayers instead of five, so that the amphiphiles pack as densely per
+    membrane area (~8.7 per unit area) as the reference's own inputs (9,600
+    in 31x30, test/mpi_bonded/result_mpi/31x31/run_param.txt).  Its
+    main loop replays mdpd_run between observation points and prints its
+    own wall time; `value` is beads * steps / that time.  --chem adds the
+    RandomChemEvent branch from the first main-loop step on."""
+    import subprocess
+    import tempfile
+    from implicit_dpd_amd import configs
+    from implicit_dpd_amd.fpdpd import FPDPD_DTYPE, write_xyz
+    p, st = configs.membrane_stack(n_chain=args.chains, n_layers=3)
+    n = len(st["pos"])
+    d = tempfile.mkdtemp(prefix="mdpd_driver_")  # a 250 MB input: host scratch, not results
+    box = tuple(p.box[:])
+    steps = max(args.steps, 1)
+    configs.write_run_param(d, box, args.chains, sol_num=0, dt=0.005, all_time=steps + 1,
+                            step_mic=10 ** 9, step_mac=100, p_thresld=2e-3, eps=0.2,
+                            max_amp_num=args.chains + 8192, influ_rad=3.0, influ_hei=1.0,
+                            influ_dep=3.0,
+                            influ_grd=0.0, core=(0, 1, 2, 3))
+    rec = np.zeros(n, dtype=FPDPD_DTYPE)
+    rec["pos"], rec["vel"], rec["vel_buf"] = st["pos"], st["vel"], st["vel"]
+    rec["id"], rec["prop"], rec["amp_id"], rec["unit"] = st["ids"], st["prop"], st["amp_id"], st["unit"]
+    log(f"[bench] writing init_config.xyz ({n} beads) to {d}")
+    write_xyz(os.path.join(d, "init_config.xyz"), rec, 0)
+    exe = os.path.join(ROOT, "implicit_dpd_amd", "bin", "mdpd_main")
+    cmd = [exe, d, "--seed", "77", "--nlist-width", "192"] + (
+        ["--chem", "--beg-chem", "0"] if args.chem else [])
+    log("[bench] " + " ".join(cmd))
+    try:
+        t = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t
+    finally:
+        for f in os.listdir(d):
+            os.remove(os.path.join(d, f))
+        os.rmdir(d)
+    if r.returncode != 0:
+        raise SystemExit(f"driver failed: {r.stderr[-2000:]}")
+    line = [x for x in r.stdout.splitlines() if x.startswith("main loop:")][-1]
+    loop_steps, loop_s = int(line.split()[2]), float(line.split()[5])
+    ms = loop_s * 1e3 / loop_steps
+    print(json.dumps({
+        "metric": "drop-in driver bead-steps/s (mdpd_main main loop, main.cpp control flow)",
+        "value": round(n / (ms * 1e-3), 1), "unit": "bead-steps/s", "n_gpus": 1,
+        "steps": loop_steps, "warmup": 1, "ms_per_step": round(ms, 5), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"lamellar membrane (3 bilayers), {args.chains} H1T3 chains ({n} beads), box "
+                               "100x100x40, the reference's multibody model from run_param.txt "
+                               "(chi 30, kappa 50, rho_co 25, cf_r 3, dt 0.005), step_mac 100, "
+                               "pair-list width 192"
+                               + (", RandomChemEvent every step (4 cores, p_thresld 2e-3)"
+                                  if args.chem else ""),
+                   "process_wall_s": round(wall, 2)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +428,10 @@ def main():
     ap.add_argument("--traj-io", action="store_true",
                     help="time trajectory output instead: one xyz frame of the membrane "
                          "written asynchronously, and steps with a frame in flight")
+    ap.add_argument("--driver", action="store_true",
+                    help="time the drop-in program mdpd_main on the headline state instead")
+    ap.add_argument("--chem", action="store_true",
+                    help="with --driver: the RandomChemEvent branch on from the first step")
     ap.add_argument("--calib-bytes", type=float, default=0.0,
                     help="only run the copy-kernel probe of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -388,6 +454,9 @@ def main():
         return
     if args.traj_io:
         traj_io_bench(args, Engine)
+        return
+    if args.driver:
+        driver_bench(args, Engine)
         return
 
     w = workload(args.workload, args.chains)

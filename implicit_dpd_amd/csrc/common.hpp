@@ -86,6 +86,9 @@ __device__ __forceinline__ int find_rep(const uint32_t* __restrict__ off, int n_
 // XCD-aware block order: the dispatcher hands consecutive blocks to the 8
 // XCDs round robin; this gives each XCD a contiguous run of tiles so that
 // neighbouring tiles share that XCD's L2.
+// nb = the tiles that hold beads; the grid may be larger (grow mode launches
+// over the capacity): its blocks past those the tiles need get tile nb, which
+// holds no bead.  With a grid of exactly nb blocks every block gets a tile.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   constexpr uint32_t X = 8;
   const uint32_t per = nb / X, rem = nb % X;
@@ -93,7 +96,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   // XCDs 0..rem-1 hold per+1 tiles, the others per
   const uint32_t base = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
   const uint32_t cnt = x < rem ? per + 1 : per;
-  return (k < cnt) ? base + k : b;  // tail (only when nb % X != 0 spills) keeps identity
+  return (k < cnt) ? base + k : nb;
 }
 
 }  // namespace mdpd

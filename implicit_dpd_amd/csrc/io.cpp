@@ -8,11 +8,16 @@
 // copies them into pinned host memory on the context's stream and returns;
 // a host thread waits for that copy, formats the lines in parallel chunks
 // and writes the file, while the caller goes on stepping.  The next write,
-// mdpd_io_wait and mdpd_destroy wait for it.
+// mdpd_io_wait and mdpd_destroy wait for it.  The writer thread makes no HIP
+// call: a host function enqueued behind the copy signals it, so it cannot
+// disturb a capture of the context's stream (mdpd_run records its steps as
+// graphs on that stream) that the caller starts meanwhile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cctype>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,7 +33,9 @@ struct XyzWriter {
   XyzRec* dev = nullptr;   // device staging, upload order
   XyzRec* host = nullptr;  // pinned copy
   uint32_t cap = 0;        // records both buffers hold
-  hipEvent_t ev = nullptr;
+  std::mutex mu;
+  std::condition_variable cv;
+  bool copied = false;     // the pinned copy of the current write is complete
   std::thread thr;
   int status = MDPD_OK;    // of the last finished write
   std::string err;
@@ -54,12 +61,21 @@ void format_rows(const XyzRec* r, uint32_t b, uint32_t e, std::string& out) {
   }
 }
 
+// enqueued behind the copy on the context's stream (no HIP call in here)
+void xyz_copied(void* p) {
+  XyzWriter* w = static_cast<XyzWriter*>(p);
+  {
+    std::lock_guard<std::mutex> lk(w->mu);
+    w->copied = true;
+  }
+  w->cv.notify_all();
+}
+
 // the writer thread: wait for the copy, format, write
 void write_job(XyzWriter* w, std::string path, uint32_t n, uint32_t time, bool append) {
-  if (hipEventSynchronize(w->ev) != hipSuccess) {
-    w->status = MDPD_ERR_HIP;
-    w->err = "trajectory copy failed";
-    return;
+  {
+    std::unique_lock<std::mutex> lk(w->mu);
+    w->cv.wait(lk, [w] { return w->copied; });
   }
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const uint32_t nt = std::max(1u, std::min({16u, hw, (n + 16383u) / 16384u}));
@@ -102,7 +118,6 @@ void xyz_writer_free(XyzWriter* w) {
   if (w->thr.joinable()) w->thr.join();
   if (w->dev) (void)hipFree(w->dev);
   if (w->host) (void)hipHostFree(w->host);
-  if (w->ev) (void)hipEventDestroy(w->ev);
   delete w;
 }
 
@@ -116,10 +131,7 @@ int mdpd_write_xyz_async(mdpd_ctx* c, const char* path, uint32_t time, int32_t a
   if (!c || !path) return fail(c, MDPD_ERR_ARG, "null argument");
   int rc = join(c);  // the previous write (and its error, if it failed)
   if (rc) return rc;
-  if (!c->xyz) {
-    c->xyz = new XyzWriter();
-    HIPCHK(c, hipEventCreateWithFlags(&c->xyz->ev, hipEventDisableTiming));
-  }
+  if (!c->xyz) c->xyz = new XyzWriter();
   XyzWriter* w = c->xyz;
   const DevState& d = c->d;
   const uint32_t n = d.n;
@@ -137,7 +149,11 @@ int mdpd_write_xyz_async(mdpd_ctx* c, const char* path, uint32_t time, int32_t a
   launch_xyz_gather(d, c->drifted ? d.vbuf : d.vel, w->dev, c->st);
   HIPCHK(c, hipGetLastError());
   if (n) HIPCHK(c, hipMemcpyAsync(w->host, w->dev, sizeof(XyzRec) * (size_t)n, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipEventRecord(w->ev, c->st));
+  {
+    std::lock_guard<std::mutex> lk(w->mu);
+    w->copied = false;
+  }
+  HIPCHK(c, hipLaunchHostFunc(c->st, xyz_copied, w));
   w->status = MDPD_OK;
   w->thr = std::thread(write_job, w, std::string(path), n, time, append != 0);
   return MDPD_OK;

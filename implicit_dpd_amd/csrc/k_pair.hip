@@ -87,7 +87,7 @@ __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s)
 template <bool MULTI>
 __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
   if (*d.rebuild == 0u) return;
-  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, blockDim.x)) * blockDim.x + threadIdx.x;
   if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
@@ -177,7 +177,7 @@ template <int ND, bool MULTI, bool RDLE1, int BS>
 __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
   __shared__ uint2 dens_queue[kDensQ * BS];  // [kDensQ][BS]
   if (halted(d)) return;
-  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, BS)) * BS + threadIdx.x;
   if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
     if (d.rebuild[0]) d.rebuild[1] += 1u;
     d.rebuild[0] = 0u;
   }
-  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, blockDim.x)) * blockDim.x + threadIdx.x;
   if (s >= dev_n(d)) return;
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     }
     __syncthreads();
   }
-  const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, blockDim.x)) * blockDim.x + threadIdx.x;
   const bool active = s < dev_n(d);
   // with keys every lane of the wave reaches the aggregated histogram
   // increment at the end

@@ -97,6 +97,12 @@ __device__ __forceinline__ uint32_t dev_n(const DevState& d) {
 __device__ __forceinline__ uint32_t dev_namp(const DevState& d) {
   return d.run_ok ? d.amp_off[d.n_rep] : d.n_amp;
 }
+// tiles of bs beads that hold live beads: the grid's size, or in grow mode
+// (grid over the capacity) those of the device's count, so that the XCD
+// mapping spreads the live tiles over all XCDs
+__device__ __forceinline__ uint32_t live_tiles(const DevState& d, uint32_t bs) {
+  return d.run_ok ? (dev_n(d) + bs - 1u) / bs : gridDim.x;
+}
 // a chemistry stop ended the run: the step's kernels do nothing
 __device__ __forceinline__ bool halted(const DevState& d) {
   return d.run_ok && *d.run_ok == 0u;

@@ -2,13 +2,23 @@
 // reference's src/main.cpp:77-209 (single process, no CALC_HEIGHT), written
 // against the C++ facade so that the calls read like the reference's:
 //
-//   mdpd_main <target_dir> [--seed S] [--chem]
+//   mdpd_main <target_dir> [--seed S] [--chem] [--beg-chem T] [--nlist-width W]
 //
 // Reads <target_dir>/run_param.txt and init_config.xyz, runs all_time steps
 // and writes kin_temp.txt, pressure.txt, diffus.txt, num_amp.txt,
 // traject.xyz and fin_config.xyz in the reference's formats.  --seed fixes
 // CalcForceEpEpDPD::m_seed (the reference takes time(nullptr), main.cpp:133);
-// --chem enables the RandomChemEvent branch (the reference's CHEM_MODE).
+// --chem enables the RandomChemEvent branch (the reference's CHEM_MODE) for
+// steps with time >= T (--beg-chem, default 100000 = Parameter::beg_chem,
+// parameter.hpp:254).  --nlist-width sets the engine's per-bead pair-list
+// capacity (default 96) for dense systems.
+//
+// The main loop runs as the engine's production path: the steps between two
+// observation points (time % step_mac == 0 or time % step_mic == 0) are one
+// mdpd_run, replayed from captured graphs, with the chemistry event inside
+// each step; the observations then read the state after the step whose time
+// they print, as main.cpp:198-208 does.
+#include <chrono>
 #include <cstring>
 #include <ctime>
 #include <iostream>
@@ -19,7 +29,6 @@
 
 namespace {
 
-constexpr uint32_t kCmCorr = 100;      // Parameter::cm_corr (parameter.hpp:203)
 constexpr uint32_t kBegChem = 100000;  // Parameter::beg_chem (parameter.hpp:254)
 
 // Parameter::RemoveCMDriftLocal (parameter.hpp:390-398)
@@ -80,9 +89,13 @@ int main(int argc, char* argv[]) {
   }
   const std::string cdir = argv[1];
   bool chem = false, have_seed = false;
-  uint32_t seed = 0;
+  uint32_t seed = 0, beg_chem = kBegChem, nlist_width = 0;
   for (int i = 2; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--chem")) chem = true;
+    else if (!std::strcmp(argv[i], "--beg-chem") && i + 1 < argc)
+      beg_chem = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
+    else if (!std::strcmp(argv[i], "--nlist-width") && i + 1 < argc)
+      nlist_width = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
     else if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) {
       seed = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
       have_seed = true;
@@ -118,6 +131,12 @@ int main(int argc, char* argv[]) {
   mdpd::ParticleSystem system(p, est_max_ptcl_num > ptcl.size() ? est_max_ptcl_num
                                                                  : (uint32_t)ptcl.size());
   system.setParticles(ptcl);
+  if (nlist_width) {
+    mdpd_tuning tu;
+    mdpd::Check(mdpd_get_tuning(system.ctx(), &tu), system.ctx(), "mdpd_get_tuning");
+    tu.nlist_width = (int32_t)nlist_width;
+    mdpd::Check(mdpd_set_tuning(system.ctx(), &tu), system.ctx(), "mdpd_set_tuning");
+  }
 
   mdpd::CalcForceEpEpDPD::m_seed = have_seed ? seed : (uint32_t)std::time(nullptr);
   mdpd::Clock::Push(system);
@@ -149,31 +168,56 @@ int main(int argc, char* argv[]) {
   cp.max_amp_num = in.max_amp_num;
   cp.n_core = in.n_core;
   cp.core_ptcl_id = core_ptcl_id.data();
-  mdpd::ChemManager chemmanag(cp);
 
   mdpd::kick(system);
   mdpd::Clock::time++;
   // End of initial step.
 
+  // the chemistry branch (main.cpp:186-196) runs inside every step of the
+  // runs below once time >= beg_chem
+  if (chem)
+    mdpd::Check(mdpd_set_chemistry(system.ctx(), &cp, beg_chem, 1), system.ctx(),
+                "mdpd_set_chemistry");
   const uint32_t atime = mdpd::Clock::time + in.all_time - 1;
-  for (; mdpd::Clock::time < atime; mdpd::Clock::time++) {
-    mdpd::drift_and_predict(system);
-    system.exchangeParticle();
-    dens_tree.calcForceAllAndWriteBack(mdpd::CalcDensity(), system);
-    force_tree.calcForceAllAndWriteBack(mdpd::CalcForceEpEpDPD(), system);
-    fbonded.CalcListedForce(system, bonded_vir);
-    mdpd::kick(system);
-    if (mdpd::Clock::time % kCmCorr == 0) mdpd::remove_cmdrift_global(system);
-    if (chem && mdpd::Clock::time >= kBegChem) {
-      if (!chemmanag.RandomChemEvent(system)) break;
+  mdpd::Clock::Push(system);
+  const uint32_t t_begin = mdpd::Clock::time;
+  const auto wall0 = std::chrono::steady_clock::now();  // timer_start (main.cpp:45-56)
+  while (mdpd::Clock::time < atime) {
+    // steps up to the next observation point, then the observations
+    uint32_t t_obs = mdpd::Clock::time;
+    while (t_obs + 1 < atime && t_obs % in.step_mac != 0 && t_obs % in.step_mic != 0) ++t_obs;
+    mdpd::Check(mdpd_run(system.ctx(), t_obs - mdpd::Clock::time + 1), system.ctx(), "mdpd_run");
+    if (chem) {
+      int32_t stopped = 0;
+      mdpd::Check(mdpd_chem_status(system.ctx(), &stopped, nullptr, nullptr), system.ctx(),
+                  "mdpd_chem_status");
+      if (stopped) {
+        // RandomChemEvent reached max_amp_num: the loop ends at that step's
+        // time, before its observations (main.cpp:188)
+        mdpd::Check(mdpd_get_clock(system.ctx(), &mdpd::Clock::time), system.ctx(), "mdpd_get_clock");
+        std::cout << "Now param.amp_num >= param.max_amp_num.\nSo, simulation will stop.\n";
+        break;
+      }
+    }
+    mdpd::Clock::time = t_obs;
+    if (t_obs % in.step_mac == 0) {
       mdpd_observables o;
       mdpd::Check(mdpd_observe(system.ctx(), &o), system.ctx(), "mdpd_observe");
-      amp_num = o.n_amp;
+      do_observe_macro(observer, system, o.n_amp, in.sol_num, o.bonded_vir);
     }
-    if (mdpd::Clock::time % in.step_mac == 0)
-      do_observe_macro(observer, system, amp_num, in.sol_num, bonded_vir);
-    if (mdpd::Clock::time % in.step_mic == 0) observer.Trajectory(system);
+    if (t_obs % in.step_mic == 0) observer.Trajectory(system);
+    mdpd::Clock::time = t_obs + 1;
   }
+
+  mdpd::Check(mdpd_sync(system.ctx()), system.ctx(), "mdpd_sync");
+  // timer_stop + show_duration, with the loop's steps and seconds for tools
+  const double wall =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+  int all_sec = (int)wall;
+  const int dd = all_sec / 86400, hh = (all_sec % 86400) / 3600, mm = (all_sec % 3600) / 60,
+            ss = all_sec % 60;
+  std::cout << "Simulation time is \n" << dd << "d " << hh << "h " << mm << "m " << ss << "s\n";
+  std::printf("main loop: %u steps in %.6f s\n", mdpd::Clock::time - t_begin, wall);
 
   observer.FinConfig(system);
   observer.CleanUp();

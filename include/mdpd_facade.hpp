@@ -346,8 +346,12 @@ class Observer {
 
  private:
   enum { kKin = 0, kPress, kDiff, kNumAmp, kTraj, kFin, kNum };
+  // one device reduction per state: the quantities of one observation share
+  // it (keyed on the engine's clock, which every step advances)
   const mdpd_observables& Observe(ParticleSystem& s) {
-    if (!have_ || obs_.time != Clock::time) {
+    uint32_t t = 0;
+    Check(mdpd_get_clock(s.ctx(), &t), s.ctx(), "mdpd_get_clock");
+    if (!have_ || obs_.time != t) {
       Check(mdpd_observe(s.ctx(), &obs_), s.ctx(), "mdpd_observe");
       have_ = true;
     }

@@ -90,6 +90,73 @@ def test_host_driver_matches_oracle(engine_mod, oracle, tmp_path):
     assert [int(x) for x in open(os.path.join(d, "num_amp.txt")).read().split()] == [n_amp] * 4
 
 
+@pytest.mark.parametrize("max_extra", [1000, 1])
+def test_host_driver_chemistry_matches_oracle(engine_mod, oracle, tmp_path, max_extra):
+    """mdpd_main --chem: the chemistry events run inside the driver's
+    mdpd_run calls from --beg-chem on; the bead records (ids, types, chains)
+    and amphiphile counts equal the oracle's step + RandomChemEvent loop, and
+    with max_amp_num one above the start the run ends at the first event
+    that inserts, as main.cpp:188 breaks (no observation of that step,
+    fin_config.xyz stamped with its time)."""
+    box = (8.0, 9.0, 8.0)
+    n_amp, n_sol, steps, beg = 500, 300, 10, 3
+    s = configs.random_chains(n_amp, box, 4, n_solvent=n_sol)
+    d = str(tmp_path)
+    kw = dict(p_thresld=0.08, eps=0.2, influ_rad=3.0, influ_hei=0.8, influ_dep=4.0, influ_grd=0.1)
+    configs.write_run_param(d, box, n_amp, sol_num=n_sol, dt=0.005, all_time=steps, step_mic=5,
+                            step_mac=3, core=(0,), max_amp_num=n_amp + max_extra, **kw)
+    n = len(s["pos"])
+    rec = np.zeros(n, dtype=FPDPD_DTYPE)
+    rec["pos"], rec["vel"], rec["vel_buf"] = s["pos"], s["vel"], s["vel"]
+    rec["id"], rec["prop"], rec["amp_id"], rec["unit"] = s["ids"], s["prop"], s["amp_id"], s["unit"]
+    write_xyz(os.path.join(d, "init_config.xyz"), rec, 0)
+    print(f"[driver] {DRIVER} {d} --seed 77 --chem --beg-chem {beg}", flush=True)
+    r = subprocess.run([DRIVER, d, "--seed", "77", "--chem", "--beg-chem", str(beg)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    p, _ = engine_mod.read_run_param(d)
+    p.pair_seed = 77
+    r0, _ = read_xyz(os.path.join(d, "init_config.xyz"))
+    L = np.array(box)
+    vel = r0["vel"] - _seq_mean(r0["vel"])
+    pos = r0["pos"] + (L * 0.5 - _seq_mean(r0["pos"]))
+    pos = pos - np.floor(pos * (1.0 / L)) * L
+    o = oracle.System(p, n + 4 * 400, n_amp + 400)
+    o.set(pos, vel, r0["id"], r0["prop"], r0["amp_id"], r0["unit"], vel_buf=r0["vel_buf"],
+          acc=r0["acc"])
+    o.set_topology(s["topo"], core_amp_id=[0])
+    cp = engine_mod.MdpdChemParams(kw["p_thresld"], kw["eps"], kw["influ_rad"], kw["influ_hei"],
+                                   kw["influ_dep"], kw["influ_grd"], n_amp + max_extra, 1,
+                                   engine_mod._u(np.zeros(1, np.uint32)))
+    amp = [n_amp]            # num_amp.txt: the initial step's observation at time 0
+    t_end = steps
+    for t in range(steps):
+        o.set_clock(t)
+        o.run(1)
+        if t >= beg and t >= 1:
+            o.set_clock(t)
+            k, stop = o.chem_event(cp)
+            if stop:
+                t_end = t
+                break
+        if t % 3 == 0 and t > 0:
+            amp.append(int(o.observe(sol_num=n_sol).n_amp))
+    fin, tf = read_xyz(os.path.join(d, "fin_config.xyz"))
+    assert tf == t_end
+    ref = o.get()
+    assert len(fin) == len(ref["id"]) > n
+    order = np.argsort(ref["id"])
+    for k in ("id", "prop", "amp_id", "unit"):
+        assert np.array_equal(fin[k], ref[k][order]), k
+    dx = fin["pos"] - ref["pos"][order]
+    dx -= np.round(dx / L) * L
+    assert np.abs(dx).max() < 1e-5, np.abs(dx).max()
+    got = [int(x) for x in open(os.path.join(d, "num_amp.txt")).read().split()]
+    assert got == amp, (got, amp)
+    if max_extra == 1:
+        assert t_end < steps - 1 and "simulation will stop" in r.stdout
+
+
 def test_autotuned_launches_are_bit_identical(engine_mod):
     box = (10.0, 10.0, 8.0)
     p = configs.warren_params(box, A=configs.MEMBRANE_A, B=25.0, rd=0.75, n_species=3, pair_seed=4)
