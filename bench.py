@@ -156,12 +156,16 @@ def cpu_baseline(p, s, topo, budget_s, max_steps, threads=None):
 
 def calibration_record():
     """The committed record of the surrogate-guided calibration against plain
-    sweeps (the metric's third clause), made by implicit_dpd_amd.calibrate."""
-    path = os.path.join(ROOT, "profiles", "calibration_runs.json")
+    sweeps (the metric's third clause), made by implicit_dpd_amd.calibrate:
+    the JSON block of profiles/calibration_runs.md."""
+    path = os.path.join(ROOT, "profiles", "calibration_runs.md")
     if not os.path.exists(path):
         return None
-    with open(path) as f:
-        return json.load(f)
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    return json.loads(block)
 
 
 class Workload:

@@ -14,7 +14,7 @@ P_zz = 0 (the coexistence condition).
   run is within 1% or passes the target, then bisects the last interval.
 - The *surrogate* is trained on ensemble runs (64 replicas of U[4k, 64k]
   beads per launch with A ~ U[-60,-20], B ~ U[10,40], rd ~ U[0.65,0.85],
-  kBT = 1) and proposes A by scanning the network; each proposal is verified
+  kBT ~ U[0.8,1.2]) and proposes A by scanning the network at kBT = 1; each proposal is verified
   by a plain run, and a miss is followed by a Newton step on the measured
   residual with the network's slope (the secant once two runs exist).
 
@@ -43,13 +43,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def slab_rho(profile, bin_vol):
+def slab_rho(profile, bin_vol, vapour_frac=0.25):
     """Bulk density of a slab from a z bead-count profile: bins above half
     the peak form the slab (periodic); the two outermost bins on each side
-    are interface and dropped."""
+    are interface and dropped.  Without a dilute vapour (no bin below
+    vapour_frac of the peak: the liquid does not coexist with a vapour, or
+    has filled the box) there is no slab and the density of the homogeneous
+    fluid, the mean over all bins, is returned; so the estimate falls
+    steadily to N / V as the attraction weakens instead of tracking noise."""
     rho = np.asarray(profile, np.float64) / bin_vol
     inside = rho > 0.5 * rho.max()
-    if inside.all():
+    if inside.all() or rho.min() > vapour_frac * rho.max():
         return float(rho.mean())
     # rotate so that the slab does not wrap
     start = int(np.argmin(inside))
@@ -84,17 +88,17 @@ def measure(e, boxes, steps, every, nbins):
     return out
 
 
-def slab(A, B, rd, n, seed, rho0=6.0):
+def slab(A, B, rd, n, seed, rho0=6.0, kbt=1.0):
     lx = max(4.0, math.ceil((n / rho0) ** (1.0 / 3.0)))
     box = (lx, lx, 2.0 * lx)
     n = int(rho0 * lx * lx * lx)
-    p = configs.warren_params(box, A=A, B=B, rd=rd, dt=0.005, pair_seed=seed)
+    p = configs.warren_params(box, A=A, B=B, rd=rd, kbt=kbt, dt=0.005, pair_seed=seed)
     s = configs.random_fluid(n, box, seed, zlim=(box[2] / 4, 3 * box[2] / 4))
     return p, s, box
 
 
-def plain_run(A, B=25.0, rd=0.75, n=200000, steps=3000, every=50, seed=11):
-    p, s, box = slab(A, B, rd, n, seed)
+def plain_run(A, B=25.0, rd=0.75, n=200000, steps=3000, every=50, seed=11, kbt=1.0):
+    p, s, box = slab(A, B, rd, n, seed, kbt=kbt)
     e = Engine(p, capacity=len(s["pos"]))
     e.upload(s["pos"], s["vel"], s["ids"], s["prop"])
     e.set_clock(0, p.pair_seed)
@@ -103,7 +107,13 @@ def plain_run(A, B=25.0, rd=0.75, n=200000, steps=3000, every=50, seed=11):
     return rho, pzz
 
 
+KBT_RANGE = (0.8, 1.2)
+
+
 def ensemble_samples(n_launch, n_rep=64, nmin=4096, nmax=65536, steps=3000, every=100, seed=100):
+    """(A, B, rd, kBT) -> (rho, P_zz) samples: n_launch ensemble launches of
+    n_rep slab replicas of U[nmin, nmax] beads with the parameters uniform
+    over their ranges, kBT over KBT_RANGE."""
     X, Y = [], []
     for b in range(n_launch):
         rng = np.random.default_rng(seed + b)
@@ -112,12 +122,13 @@ def ensemble_samples(n_launch, n_rep=64, nmin=4096, nmax=65536, steps=3000, ever
             A = rng.uniform(*A_RANGE)
             B = rng.uniform(*B_RANGE)
             rd = rng.uniform(*RD_RANGE)
+            kbt = rng.uniform(*KBT_RANGE)
             n = int(rng.integers(nmin, nmax + 1))
-            p, s, box = slab(A, B, rd, n, seed=(seed + b) * 1000 + r + 1)
+            p, s, box = slab(A, B, rd, n, seed=(seed + b) * 1000 + r + 1, kbt=kbt)
             params.append(p)
             systems.append(s)
             boxes.append(box)
-            xs.append((A, B, rd, 1.0))
+            xs.append((A, B, rd, kbt))
         allb = configs.concat_systems(systems)
         e = Engine(params, n_beads=[len(s["pos"]) for s in systems])
         # strongly attractive, weakly repulsive corners of the sampled range

@@ -161,6 +161,16 @@ def test_slab_density_estimator():
     assert abs(slab_rho(prof, 1.0) - 6.0) < 1e-12
     prof2 = prof * 2.0
     assert abs(slab_rho(prof2, 2.0) - 6.0) < 1e-12
+    # no dilute vapour (the liquid filled the box or never condensed): the
+    # homogeneous fluid's density, the mean over the box
+    flat = np.full(40, 3.0)
+    flat[10:20] = 4.5
+    assert abs(slab_rho(flat, 1.0) - flat.mean()) < 1e-12
+    # the slab estimate holds while the vapour stays below a quarter of the
+    # liquid's density, and gives way to the box mean above that
+    lo = [slab_rho(np.where(np.arange(40) < 20, 6.0, v), 1.0) for v in (0.5, 1.0, 1.4)]
+    assert lo[0] == lo[1] == lo[2] == 6.0
+    assert slab_rho(np.where(np.arange(40) < 20, 6.0, 1.6), 1.0) == 3.8
 
 
 @pytest.mark.gpu
