@@ -452,8 +452,21 @@ void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_
 }
 
 // ---------------------------------------------------------------- force
-// pairs with 1 - r below this use an FP64 1 - r under the sqrt envelope
-constexpr float kNearCut = 2.0e-3f;
+// Pairs with 1 - r below these take 1 - r from the FP64 master positions.
+// The FP32 displacement carries about 1e-7 of absolute error in r: sqrt(1 - r)
+// magnifies it next to the cutoff, and under the linear envelope the random
+// force sigma' xi (1 - r) (sigma' = sigma / sqrt(dt), about 42 at dt = 0.005)
+// is off by sigma' |xi| 1e-7 absolute, more than 1e-5 of the bead's pair
+// forces when a pair within 0.025 of the cutoff dominates them (a vapour
+// bead with one neighbour).
+constexpr float kNearCutSqrt = 2.0e-3f, kNearCutLin = 2.5e-2f;
+// Such pairs are evaluated after the pair loop (so that the FP64 work runs
+// lane-dense rather than on the few lanes that meet one in an iteration):
+// the loop queues each with its coefficients, kBandQ deep per lane in LDS;
+// a lane's pairs past that are evaluated in the loop.
+constexpr uint32_t kBandQ = 4;
+// dynamic LDS of a force block of bs threads: kBandQ slots per lane
+constexpr size_t force_lds_bytes(uint32_t bs) { return (size_t)kBandQ * bs * (16 + 4); }
 
 template <int NS>
 __device__ __forceinline__ float sel(const float (&v)[NS], uint32_t t) {
@@ -502,6 +515,12 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   // with keys every lane of the wave reaches the aggregated histogram
   // increment at the end
   if (!active && !keys) return;
+  // the near-cutoff queue: coefficients [kBandQ][blockDim] then entries
+  extern __shared__ char force_lds[];
+  float4* const bq_coef = reinterpret_cast<float4*>(force_lds);
+  uint32_t* const bq_ent =
+      reinterpret_cast<uint32_t*>(force_lds + (size_t)16 * kBandQ * blockDim.x);
+  uint32_t bq_n = 0;
   uint32_t key = 0;
   if (active) {
   const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
@@ -533,6 +552,49 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   cnt = cnt < W ? cnt : W;
   const uint4* lst = tile_list(d, s);
   float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+  // a pair next to the cutoff: the FP32 direction as in the loop, 1 - r from
+  // the FP64 master positions, the coefficients the loop computed
+  // (pid: the bead's own FP64 position, loaded where needed)
+  auto band_pair = [&](uint32_t e, float4 cf, const double* pid) {
+    const uint32_t t = e & kIndexMask;
+    const uint32_t code = e >> kCodeShift;
+    const float4 pj = d.sb[t].pos;
+    float dx, dy, dz;
+    if (!MULTI) {
+      const float4 o = otab[code];
+      dx = (pi.x - o.x) - pj.x;
+      dy = (pi.y - o.y) - pj.y;
+      dz = (pi.z - o.z) - pj.z;
+    } else {
+      dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+      dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
+      dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
+    }
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    // 1 - r = (1 - r^2) / (1 + r): only the cancelling 1 - r^2 needs FP64
+    double v2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double x = pid[a] - d.pos[a * d.cap + t];
+      if (x > 0.5 * R.box[a]) x -= R.box[a];
+      else if (x < -0.5 * R.box[a]) x += R.box[a];
+      v2 += x * x;
+    }
+    const float wc = fmaxf((float)(1.0 - v2) / (1.0f + r2 * rinv), 0.0f);
+    float wR, wD;
+    if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
+    else { wR = wc; wD = wc * wc; }
+    const float fs = (cf.x * wc + cf.y + cf.z * wR - cf.w * wD) * rinv;
+    fx += fs * dx;
+    fy += fs * dy;
+    fz += fs * dz;
+    if (VIRIAL) {
+      px += fs * dx * dx;
+      py += fs * dy * dy;
+      pz += fs * dz * dz;
+    }
+  };
   auto pair = [&](uint32_t e) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
@@ -569,13 +631,9 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     const uint32_t idj = wj >> 2, tj = NS == 1 ? 0u : (wj & 3u);
     Saru sr = saru_seed(idi < idj ? idi : idj, idi < idj ? idj : idi, seed3);
     const float xi = saru_nrml_f32(sr);
-    float wc = fmaxf(1.0f - rr, 0.0f);
-    if (ENV == 0 && wc < kNearCut) {
-      // sqrt(1 - r) magnifies the FP32 error of r next to the cutoff; take
-      // 1 - r there from the FP64 master positions
-      const double r2d = fp64_r2(d, R, s, t);
-      wc = (float)fmax((1.0 - r2d) / (1.0 + sqrt(r2d)), 0.0);
-    }
+    const float wc = fmaxf(1.0f - rr, 0.0f);
+    // next to the cutoff 1 - r must come from the FP64 master positions
+    const bool band = wc < (ENV == 0 ? kNearCutSqrt : kNearCutLin);
     float wR, wD;
     if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
     else { wR = wc; wD = wc * wc; }
@@ -616,6 +674,25 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     // envelope (whose wc may come from FP64 near the cutoff) needs wc itself
     const float wm = (ENV == 0 && rdm1) ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
     const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+    if (band) {
+      // queued with its coefficients for the FP64 pass after the loop:
+      // fs = (K1 wc + K0 + sgxi wR - gmrv wD) rinv with the FP64 wc
+      const bool mw = ENV == 0 && rdm1;  // the many-body weight is wc itself
+      const float4 cf = make_float4(mw ? cons_c + mbs : cons_c, mw ? 0.f : mbs * wm, sg * xi,
+                                    gm * rv * rinv);
+      if (bq_n < kBandQ) {
+        const uint32_t slot = bq_n * blockDim.x + threadIdx.x;
+        bq_coef[slot] = cf;
+        bq_ent[slot] = e;
+        ++bq_n;
+      } else {  // the lane's queue is full (rare): evaluate it here
+        double pid[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
+        band_pair(e, cf, pid);
+      }
+      return;
+    }
     const float fs = (cons_c * wc + mbs * wm + sg * wR * xi - gm * wD * rv * rinv) * rinv;
     fx += fs * dx;
     fy += fs * dy;
@@ -632,6 +709,13 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     if (k + 1 < cnt) pair(ch.y);
     if (k + 2 < cnt) pair(ch.z);
     if (k + 3 < cnt) pair(ch.w);
+  }
+  if (bq_n) {
+    double pid[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
+    for (uint32_t k = 0; k < bq_n; ++k)
+      band_pair(bq_ent[k * blockDim.x + threadIdx.x], bq_coef[k * blockDim.x + threadIdx.x], pid);
   }
   if (VIRIAL) {
     d.press[s] = px;
@@ -679,13 +763,14 @@ template <int NS, int ND, int ENV>
 static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb,
                           int cv, int ky) {
   const dim3 g(div_up(d.nl, block)), b(block);
+  const size_t lds = force_lds_bytes((uint32_t)block);
   const bool multi = d.n_rep > 1;
   if (multi) {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv, ky);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb, cv, ky);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, lds, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, lds, st, d, vb, cv, ky);
   } else {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb, cv, ky);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb, cv, ky);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, lds, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, lds, st, d, vb, cv, ky);
   }
 }
 

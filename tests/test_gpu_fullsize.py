@@ -1,6 +1,7 @@
 """GPU parity at the benchmark configurations' full sizes.
 
-- Membrane (1,048,576 beads, the headline system), MDPD slab (198,246) and
+- Membrane (1,048,576 beads as stacked bilayers, the headline system; and
+  the same model on random chains), MDPD slab (198,246) and
   the 64-replica ensemble (about 2.2M beads), after 20 steps of the
   production path: the pair sets of the step (cutoff 1, the pair list the
   force pass walks) and of rd equal the FP64 oracle's cell-list pair sets
@@ -60,11 +61,16 @@ def _after_steps(e, steps):
     e.force()
 
 
-@pytest.mark.parametrize("name", ["membrane", "slab"])
+FULL = {"membrane_stack": configs.membrane_stack, "membrane_gas": configs.membrane,
+        "slab": configs.mdpd_slab}
+
+
+@pytest.mark.parametrize("name", ["membrane_stack", "membrane_gas", "slab"])
 def test_full_size_pairs_density_force_match_oracle(engine_mod, oracle, name):
-    """Membrane (1,048,576 beads, 3 species, chains) and MDPD slab (198,246
-    beads, liquid + vapour) after 20 production steps."""
-    p, s = configs.membrane() if name == "membrane" else configs.mdpd_slab()
+    """The headline membrane (1,048,576 beads as five stacked bilayers, 3
+    species, chains), the same model on random chains, and the MDPD slab
+    (198,246 beads, liquid + vapour), after 20 production steps."""
+    p, s = FULL[name]()
     n = len(s["pos"])
     e = engine_mod.Engine(p, capacity=n)
     e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
