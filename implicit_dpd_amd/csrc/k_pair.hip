@@ -460,13 +460,58 @@ void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_
 // forces when a pair within 0.025 of the cutoff dominates them (a vapour
 // bead with one neighbour).
 constexpr float kNearCutSqrt = 2.0e-3f, kNearCutLin = 2.5e-2f;
-// Such pairs are evaluated after the pair loop (so that the FP64 work runs
-// lane-dense rather than on the few lanes that meet one in an iteration):
-// the loop queues each with its coefficients, kBandQ deep per lane in LDS;
-// a lane's pairs past that are evaluated in the loop.
+// Such pairs are evaluated after the pair loop, spread over the lanes of the
+// wave: the loop queues each with its coefficients, kBandQ deep per lane in
+// LDS (a lane's pairs past that are evaluated in the loop); the wave then
+// numbers its queued pairs owner by owner, each lane evaluates every 64th,
+// writes the pair's (fs, dx, dy, dz) over its coefficients, and every owner
+// adds its own pairs in queue order.  A wave meets about one such pair per
+// bead (7% of the pairs under the linear envelope), so this takes one or two
+// passes instead of one per queued pair of the busiest lane.
 constexpr uint32_t kBandQ = 4;
-// dynamic LDS of a force block of bs threads: kBandQ slots per lane
-constexpr size_t force_lds_bytes(uint32_t bs) { return (size_t)kBandQ * bs * (16 + 4); }
+// dynamic LDS of a force block of bs threads: kBandQ slots per lane (16-byte
+// coefficients + 4-byte entry) and the wave's numbering (one byte per slot)
+constexpr size_t force_lds_bytes(uint32_t bs) { return (size_t)kBandQ * bs * (16 + 4 + 1); }
+
+// One queued near-cutoff pair of bead s (FP32 cell-local position pi, FP64
+// master position pid) and list entry e: the FP32 direction as in the pair
+// loop, 1 - r from the FP64 master positions, the coefficients cf the loop
+// computed: fs = (K1 wc + K0 + sgxi wR - gmrv wD) rinv.  Returns (fs, dx, dy, dz).
+template <int ENV, bool MULTI>
+__device__ __forceinline__ float4 band_pair(const DevState& d, const Replica& R, float4 pi,
+                                            const double* pid, uint32_t e, float4 cf,
+                                            const float4* otab) {
+  const uint32_t t = e & kIndexMask;
+  const uint32_t code = e >> kCodeShift;
+  const float4 pj = d.sb[t].pos;
+  float dx, dy, dz;
+  if (!MULTI) {
+    const float4 o = otab[code];
+    dx = (pi.x - o.x) - pj.x;
+    dy = (pi.y - o.y) - pj.y;
+    dz = (pi.z - o.z) - pj.z;
+  } else {
+    dx = (pi.x - (float)((int)(code & 3u) - 1) * R.hf[0]) - pj.x;
+    dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * R.hf[1]) - pj.y;
+    dz = (pi.z - (float)((int)(code >> 4) - 1) * R.hf[2]) - pj.z;
+  }
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  // 1 - r = (1 - r^2) / (1 + r): only the cancelling 1 - r^2 needs FP64
+  double v2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double x = pid[a] - d.pos[a * d.cap + t];
+    if (x > 0.5 * R.box[a]) x -= R.box[a];
+    else if (x < -0.5 * R.box[a]) x += R.box[a];
+    v2 += x * x;
+  }
+  const float wc = fmaxf((float)(1.0 - v2) / (1.0f + r2 * rinv), 0.0f);
+  float wR, wD;
+  if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
+  else { wR = wc; wD = wc * wc; }
+  return make_float4((cf.x * wc + cf.y + cf.z * wR - cf.w * wD) * rinv, dx, dy, dz);
+}
 
 template <int NS>
 __device__ __forceinline__ float sel(const float (&v)[NS], uint32_t t) {
@@ -512,18 +557,22 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   }
   const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, blockDim.x)) * blockDim.x + threadIdx.x;
   const bool active = s < dev_n(d);
-  // with keys every lane of the wave reaches the aggregated histogram
-  // increment at the end
-  if (!active && !keys) return;
-  // the near-cutoff queue: coefficients [kBandQ][blockDim] then entries
+  // every lane of the wave stays to the end: it may evaluate near-cutoff
+  // pairs of its neighbours, and with keys it reaches the aggregated
+  // histogram increment
+  // the near-cutoff queue: coefficients [kBandQ][blockDim], entries
+  // [kBandQ][blockDim], then the numbering [blockDim / 64][kBandQ * 64]
   extern __shared__ char force_lds[];
   float4* const bq_coef = reinterpret_cast<float4*>(force_lds);
   uint32_t* const bq_ent =
       reinterpret_cast<uint32_t*>(force_lds + (size_t)16 * kBandQ * blockDim.x);
+  uint8_t* const bq_own = reinterpret_cast<uint8_t*>(force_lds + (size_t)20 * kBandQ * blockDim.x) +
+                          (threadIdx.x & ~63u) * kBandQ;
   uint32_t bq_n = 0;
   uint32_t key = 0;
+  const int r = active ? (MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0) : 0;
+  float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
   if (active) {
-  const int r = MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0;
   const Replica& R = d.reps[r];
   const uint32_t seed3 = R.pair_seed + *d.d_time;
   const float4 pi = d.sb[s].pos;
@@ -551,50 +600,6 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   uint32_t cnt = d.ncount[s];
   cnt = cnt < W ? cnt : W;
   const uint4* lst = tile_list(d, s);
-  float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-  // a pair next to the cutoff: the FP32 direction as in the loop, 1 - r from
-  // the FP64 master positions, the coefficients the loop computed
-  // (pid: the bead's own FP64 position, loaded where needed)
-  auto band_pair = [&](uint32_t e, float4 cf, const double* pid) {
-    const uint32_t t = e & kIndexMask;
-    const uint32_t code = e >> kCodeShift;
-    const float4 pj = d.sb[t].pos;
-    float dx, dy, dz;
-    if (!MULTI) {
-      const float4 o = otab[code];
-      dx = (pi.x - o.x) - pj.x;
-      dy = (pi.y - o.y) - pj.y;
-      dz = (pi.z - o.z) - pj.z;
-    } else {
-      dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
-      dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
-      dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
-    }
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    // 1 - r = (1 - r^2) / (1 + r): only the cancelling 1 - r^2 needs FP64
-    double v2 = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double x = pid[a] - d.pos[a * d.cap + t];
-      if (x > 0.5 * R.box[a]) x -= R.box[a];
-      else if (x < -0.5 * R.box[a]) x += R.box[a];
-      v2 += x * x;
-    }
-    const float wc = fmaxf((float)(1.0 - v2) / (1.0f + r2 * rinv), 0.0f);
-    float wR, wD;
-    if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
-    else { wR = wc; wD = wc * wc; }
-    const float fs = (cf.x * wc + cf.y + cf.z * wR - cf.w * wD) * rinv;
-    fx += fs * dx;
-    fy += fs * dy;
-    fz += fs * dz;
-    if (VIRIAL) {
-      px += fs * dx * dx;
-      py += fs * dy * dy;
-      pz += fs * dz * dz;
-    }
-  };
   auto pair = [&](uint32_t e) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
@@ -689,7 +694,15 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
         double pid[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
-        band_pair(e, cf, pid);
+        const float4 q = band_pair<ENV, MULTI>(d, R, pi, pid, e, cf, otab);
+        fx += q.x * q.y;
+        fy += q.x * q.z;
+        fz += q.x * q.w;
+        if (VIRIAL) {
+          px += q.x * q.y * q.y;
+          py += q.x * q.z * q.z;
+          pz += q.x * q.w * q.w;
+        }
       }
       return;
     }
@@ -710,13 +723,54 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     if (k + 2 < cnt) pair(ch.z);
     if (k + 3 < cnt) pair(ch.w);
   }
-  if (bq_n) {
-    double pid[3];
+  }  // active
+  {
+    // the wave's queued near-cutoff pairs, numbered owner by owner: bits b
+    // of the per-lane counts (at most kBandQ = 4) give each lane's offset
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
-    for (uint32_t k = 0; k < bq_n; ++k)
-      band_pair(bq_ent[k * blockDim.x + threadIdx.x], bq_coef[k * blockDim.x + threadIdx.x], pid);
+    for (int b = 0; b < 3; ++b) {
+      const unsigned long long m = __ballot((bq_n >> b) & 1u);
+      pre += __mbcnt_hi((uint32_t)(m >> 32), __mbcnt_lo((uint32_t)m, 0u)) << b;
+      tot += (uint32_t)__popcll(m) << b;
+    }
+    if (tot) {
+      for (uint32_t j = 0; j < bq_n; ++j) bq_own[pre + j] = (uint8_t)(lane << 2 | j);
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t s0 = s - lane;  // the wave's first slot
+      for (uint32_t g0 = 0; g0 < tot; g0 += 64u) {
+        const uint32_t g = g0 + lane;
+        const uint32_t ow = g < tot ? bq_own[g] : 0u;
+        const uint32_t owner = ow >> 2;
+        const uint32_t slot = (ow & 3u) * blockDim.x + (threadIdx.x & ~63u) + owner;
+        const int ro = MULTI ? __shfl(r, (int)owner) : 0;
+        if (g < tot) {
+          const uint32_t so = s0 + owner;
+          const Replica& Ro = d.reps[ro];
+          double pid[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + so];
+          bq_coef[slot] =
+              band_pair<ENV, MULTI>(d, Ro, d.sb[so].pos, pid, bq_ent[slot], bq_coef[slot], otab);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t j = 0; j < bq_n; ++j) {
+        const float4 q = bq_coef[j * blockDim.x + threadIdx.x];
+        fx += q.x * q.y;
+        fy += q.x * q.z;
+        fz += q.x * q.w;
+        if (VIRIAL) {
+          px += q.x * q.y * q.y;
+          py += q.x * q.z * q.z;
+          pz += q.x * q.w * q.w;
+        }
+      }
+    }
   }
+  if (active) {
+  const Replica& R = d.reps[r];
   if (VIRIAL) {
     d.press[s] = px;
     d.press[d.cap + s] = py;
