@@ -452,52 +452,32 @@ void launch_density(const DevState& d, int /*ns*/, int nd, int block, hipStream_
 }
 
 // ---------------------------------------------------------------- force
-// Pairs with 1 - r below these take 1 - r from the FP64 master positions.
-// The FP32 displacement carries about 1e-7 of absolute error in r: sqrt(1 - r)
-// magnifies it next to the cutoff, and under the linear envelope the random
-// force sigma' xi (1 - r) (sigma' = sigma / sqrt(dt), about 42 at dt = 0.005)
-// is off by sigma' |xi| 1e-7 absolute, more than 1e-5 of the bead's pair
-// forces when a pair within 0.025 of the cutoff dominates them (a vapour
-// bead with one neighbour).
-constexpr float kNearCutSqrt = 2.0e-3f, kNearCutLin = 2.5e-2f;
-// Such pairs are evaluated after the pair loop, spread over the lanes of the
-// wave: the loop queues each with its coefficients, kBandQ deep per lane in
-// LDS (a lane's pairs past that are evaluated in the loop); the wave then
-// numbers its queued pairs owner by owner, each lane evaluates every 64th,
-// writes the pair's (fs, dx, dy, dz) over its coefficients, and every owner
-// adds its own pairs in queue order.  A wave meets about one such pair per
-// bead (7% of the pairs under the linear envelope), so this takes one or two
-// passes instead of one per queued pair of the busiest lane.
-constexpr uint32_t kBandQ = 4;
-// dynamic LDS of a force block of bs threads: kBandQ slots per lane (16-byte
-// coefficients + 4-byte entry) and the wave's numbering (one byte per slot)
-constexpr size_t force_lds_bytes(uint32_t bs) { return (size_t)kBandQ * bs * (16 + 4 + 1); }
+// Near the cutoff the FP32 value of 1 - r can decide whether a bead's force
+// meets 1e-5 of its pair-force sum.  The FP32 displacement carries about
+// 1e-7 of absolute error in r (kDrEst bounds it with a margin of 2.5), and a
+// pair's force term moves by its coefficient times that error over the
+// derivative of its weight: relative to the term, 1e-7 / (1 - r) under the
+// linear weights and 1e-7 / (2 (1 - r)) under sqrt(1 - r), above 4e-6 only
+// within 0.025 of the cutoff.  There (the band: 7% of the pairs) a bead
+// with few other neighbours can miss the bar: under the linear envelope the
+// random force sigma' xi (1 - r) (sigma' = sigma / sqrt(dt), about 42 at
+// dt = 0.005) is off by sigma' |xi| 1e-7 absolute, more than 1e-5 of the
+// forces of a vapour bead with one neighbour.  The pair loop sums, over the
+// band pairs, a bound of |dF/dr| (the random, conservative and dissipative
+// coefficients times the derivatives of their weights) and, over all pairs,
+// |F_ij|; a bead whose band error estimate kDrEst * sum |dF/dr| exceeds half
+// of 1e-5 * max(sum |F_ij|, 1e-3) walks its list again with 1 - r of every
+// band pair from the FP64 master positions.  Only isolated beads (vapour, a
+// few neighbours) fail the test; in a liquid the band error is orders of
+// magnitude below the bar.
+constexpr float kNearCut = 2.5e-2f;
+constexpr float kDrEst = 2.5e-7f;
 
-// One queued near-cutoff pair of bead s (FP32 cell-local position pi, FP64
-// master position pid) and list entry e: the FP32 direction as in the pair
-// loop, 1 - r from the FP64 master positions, the coefficients cf the loop
-// computed: fs = (K1 wc + K0 + sgxi wR - gmrv wD) rinv.  Returns (fs, dx, dy, dz).
-template <int ENV, bool MULTI>
-__device__ __forceinline__ float4 band_pair(const DevState& d, const Replica& R, float4 pi,
-                                            const double* pid, uint32_t e, float4 cf,
-                                            const float4* otab) {
-  const uint32_t t = e & kIndexMask;
-  const uint32_t code = e >> kCodeShift;
-  const float4 pj = d.sb[t].pos;
-  float dx, dy, dz;
-  if (!MULTI) {
-    const float4 o = otab[code];
-    dx = (pi.x - o.x) - pj.x;
-    dy = (pi.y - o.y) - pj.y;
-    dz = (pi.z - o.z) - pj.z;
-  } else {
-    dx = (pi.x - (float)((int)(code & 3u) - 1) * R.hf[0]) - pj.x;
-    dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * R.hf[1]) - pj.y;
-    dz = (pi.z - (float)((int)(code >> 4) - 1) * R.hf[2]) - pj.z;
-  }
-  const float r2 = dx * dx + dy * dy + dz * dz;
-  const float rinv = __builtin_amdgcn_rsqf(r2);
-  // 1 - r = (1 - r^2) / (1 + r): only the cancelling 1 - r^2 needs FP64
+// 1 - r of the pair (s, t) from the FP64 master positions, minimum image,
+// for the FP32 r = rr: 1 - r = (1 - r^2) / (1 + r), where only the
+// cancelling 1 - r^2 needs FP64
+__device__ __forceinline__ float fp64_one_minus_r(const DevState& d, const Replica& R,
+                                                  const double* pid, uint32_t t, float rr) {
   double v2 = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -506,11 +486,7 @@ __device__ __forceinline__ float4 band_pair(const DevState& d, const Replica& R,
     else if (x < -0.5 * R.box[a]) x += R.box[a];
     v2 += x * x;
   }
-  const float wc = fmaxf((float)(1.0 - v2) / (1.0f + r2 * rinv), 0.0f);
-  float wR, wD;
-  if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
-  else { wR = wc; wD = wc * wc; }
-  return make_float4((cf.x * wc + cf.y + cf.z * wR - cf.w * wD) * rinv, dx, dy, dz);
+  return fmaxf((float)(1.0 - v2) / (1.0f + rr), 0.0f);
 }
 
 template <int NS>
@@ -557,21 +533,11 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   }
   const uint32_t s = xcd_block(blockIdx.x, live_tiles(d, blockDim.x)) * blockDim.x + threadIdx.x;
   const bool active = s < dev_n(d);
-  // every lane of the wave stays to the end: it may evaluate near-cutoff
-  // pairs of its neighbours, and with keys it reaches the aggregated
-  // histogram increment
-  // the near-cutoff queue: coefficients [kBandQ][blockDim], entries
-  // [kBandQ][blockDim], then the numbering [blockDim / 64][kBandQ * 64]
-  extern __shared__ char force_lds[];
-  float4* const bq_coef = reinterpret_cast<float4*>(force_lds);
-  uint32_t* const bq_ent =
-      reinterpret_cast<uint32_t*>(force_lds + (size_t)16 * kBandQ * blockDim.x);
-  uint8_t* const bq_own = reinterpret_cast<uint8_t*>(force_lds + (size_t)20 * kBandQ * blockDim.x) +
-                          (threadIdx.x & ~63u) * kBandQ;
-  uint32_t bq_n = 0;
+  // with keys every lane of the wave reaches the aggregated histogram
+  // increment at the end
+  if (!active && !keys) return;
   uint32_t key = 0;
   const int r = active ? (MULTI ? find_rep(d.rep_off, d.n_rep, s) : 0) : 0;
-  float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
   if (active) {
   const Replica& R = d.reps[r];
   const uint32_t seed3 = R.pair_seed + *d.d_time;
@@ -600,7 +566,12 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   uint32_t cnt = d.ncount[s];
   cnt = cnt < W ? cnt : W;
   const uint4* lst = tile_list(d, s);
-  auto pair = [&](uint32_t e) {
+  float fx = 0.f, fy = 0.f, fz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+  // band error estimate (sum of |dF/dr| over band pairs) and sum of |F_ij|
+  float band_e = 0.f, fsum = 0.f;
+  double pid[3];  // the bead's FP64 master position (exact walk only)
+  // exact: 1 - r of band pairs from the FP64 master positions
+  auto pair = [&](uint32_t e, bool exact) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
     const float4 pj = d.sb[t].pos;
@@ -636,9 +607,9 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     const uint32_t idj = wj >> 2, tj = NS == 1 ? 0u : (wj & 3u);
     Saru sr = saru_seed(idi < idj ? idi : idj, idi < idj ? idj : idi, seed3);
     const float xi = saru_nrml_f32(sr);
-    const float wc = fmaxf(1.0f - rr, 0.0f);
-    // next to the cutoff 1 - r must come from the FP64 master positions
-    const bool band = wc < (ENV == 0 ? kNearCutSqrt : kNearCutLin);
+    float wc = fmaxf(1.0f - rr, 0.0f);
+    const bool band = wc < kNearCut;
+    if (exact && band) wc = fp64_one_minus_r(d, R, pid, t, rr);
     float wR, wD;
     if (ENV == 0) { wR = __builtin_amdgcn_sqrtf(wc); wD = wc; }
     else { wR = wc; wD = wc * wc; }
@@ -679,34 +650,18 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     // envelope (whose wc may come from FP64 near the cutoff) needs wc itself
     const float wm = (ENV == 0 && rdm1) ? wc : fmaxf(1.0f - rr * inv_rdm, 0.0f);
     const float rv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
-    if (band) {
-      // queued with its coefficients for the FP64 pass after the loop:
-      // fs = (K1 wc + K0 + sgxi wR - gmrv wD) rinv with the FP64 wc
-      const bool mw = ENV == 0 && rdm1;  // the many-body weight is wc itself
-      const float4 cf = make_float4(mw ? cons_c + mbs : cons_c, mw ? 0.f : mbs * wm, sg * xi,
-                                    gm * rv * rinv);
-      if (bq_n < kBandQ) {
-        const uint32_t slot = bq_n * blockDim.x + threadIdx.x;
-        bq_coef[slot] = cf;
-        bq_ent[slot] = e;
-        ++bq_n;
-      } else {  // the lane's queue is full (rare): evaluate it here
-        double pid[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
-        const float4 q = band_pair<ENV, MULTI>(d, R, pi, pid, e, cf, otab);
-        fx += q.x * q.y;
-        fy += q.x * q.z;
-        fz += q.x * q.w;
-        if (VIRIAL) {
-          px += q.x * q.y * q.y;
-          py += q.x * q.z * q.z;
-          pz += q.x * q.w * q.w;
-        }
-      }
-      return;
+    const float sgx = sg * xi, gmr = gm * rv * rinv;
+    const float fs = (cons_c * wc + mbs * wm + sgx * wR - gmr * wD) * rinv;
+    if (!exact) {
+      // |dwR/dr| = 1 / (2 sqrt(1 - r)) or 1, |dwD/dr| = 1 or 2 (1 - r),
+      // |dwm/dr| = 1 / rdm where wm > 0 (in the band only with rdm near 1)
+      const float kR = ENV == 0 ? 0.5f * __builtin_amdgcn_rsqf(wc) : 1.0f;
+      const float kD = ENV == 0 ? 1.0f : 2.0f * wc;
+      const float de = fabsf(sgx) * kR + fabsf(gmr) * kD + fabsf(cons_c) +
+                       fabsf(mbs) * (wm > 0.0f ? inv_rdm : 0.0f);
+      band_e += band ? de : 0.0f;
+      fsum += fabsf(fs) * rr;
     }
-    const float fs = (cons_c * wc + mbs * wm + sg * wR * xi - gm * wD * rv * rinv) * rinv;
     fx += fs * dx;
     fy += fs * dy;
     fz += fs * dz;
@@ -718,59 +673,24 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   };
   for (uint32_t k = 0; k < cnt; k += 4) {
     const uint4 ch = lst[(size_t)(k >> 2) * 64u];
-    pair(ch.x);
-    if (k + 1 < cnt) pair(ch.y);
-    if (k + 2 < cnt) pair(ch.z);
-    if (k + 3 < cnt) pair(ch.w);
+    pair(ch.x, false);
+    if (k + 1 < cnt) pair(ch.y, false);
+    if (k + 2 < cnt) pair(ch.z, false);
+    if (k + 3 < cnt) pair(ch.w, false);
   }
-  }  // active
-  {
-    // the wave's queued near-cutoff pairs, numbered owner by owner: bits b
-    // of the per-lane counts (at most kBandQ = 4) give each lane's offset
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t pre = 0, tot = 0;
+  if (kDrEst * band_e > 0.5e-5f * fmaxf(fsum, 1.0e-3f)) {
+    // rare: the same walk with the band's 1 - r from FP64
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const unsigned long long m = __ballot((bq_n >> b) & 1u);
-      pre += __mbcnt_hi((uint32_t)(m >> 32), __mbcnt_lo((uint32_t)m, 0u)) << b;
-      tot += (uint32_t)__popcll(m) << b;
-    }
-    if (tot) {
-      for (uint32_t j = 0; j < bq_n; ++j) bq_own[pre + j] = (uint8_t)(lane << 2 | j);
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t s0 = s - lane;  // the wave's first slot
-      for (uint32_t g0 = 0; g0 < tot; g0 += 64u) {
-        const uint32_t g = g0 + lane;
-        const uint32_t ow = g < tot ? bq_own[g] : 0u;
-        const uint32_t owner = ow >> 2;
-        const uint32_t slot = (ow & 3u) * blockDim.x + (threadIdx.x & ~63u) + owner;
-        const int ro = MULTI ? __shfl(r, (int)owner) : 0;
-        if (g < tot) {
-          const uint32_t so = s0 + owner;
-          const Replica& Ro = d.reps[ro];
-          double pid[3];
-#pragma unroll
-          for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + so];
-          bq_coef[slot] =
-              band_pair<ENV, MULTI>(d, Ro, d.sb[so].pos, pid, bq_ent[slot], bq_coef[slot], otab);
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      for (uint32_t j = 0; j < bq_n; ++j) {
-        const float4 q = bq_coef[j * blockDim.x + threadIdx.x];
-        fx += q.x * q.y;
-        fy += q.x * q.z;
-        fz += q.x * q.w;
-        if (VIRIAL) {
-          px += q.x * q.y * q.y;
-          py += q.x * q.z * q.z;
-          pz += q.x * q.w * q.w;
-        }
-      }
+    for (int a = 0; a < 3; ++a) pid[a] = d.pos[a * d.cap + s];
+    fx = fy = fz = px = py = pz = 0.f;
+    for (uint32_t k = 0; k < cnt; k += 4) {
+      const uint4 ch = lst[(size_t)(k >> 2) * 64u];
+      pair(ch.x, true);
+      if (k + 1 < cnt) pair(ch.y, true);
+      if (k + 2 < cnt) pair(ch.z, true);
+      if (k + 3 < cnt) pair(ch.w, true);
     }
   }
-  if (active) {
-  const Replica& R = d.reps[r];
   if (VIRIAL) {
     d.press[s] = px;
     d.press[d.cap + s] = py;
@@ -817,14 +737,13 @@ template <int NS, int ND, int ENV>
 static void force_launch3(const DevState& d, bool virial, int block, hipStream_t st, double* vb,
                           int cv, int ky) {
   const dim3 g(div_up(d.nl, block)), b(block);
-  const size_t lds = force_lds_bytes((uint32_t)block);
   const bool multi = d.n_rep > 1;
   if (multi) {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, lds, st, d, vb, cv, ky);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, lds, st, d, vb, cv, ky);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, true, true>), g, b, 0, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, true, false>), g, b, 0, st, d, vb, cv, ky);
   } else {
-    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, lds, st, d, vb, cv, ky);
-    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, lds, st, d, vb, cv, ky);
+    if (virial) hipLaunchKernelGGL((k_force<NS, ND, ENV, false, true>), g, b, 0, st, d, vb, cv, ky);
+    else hipLaunchKernelGGL((k_force<NS, ND, ENV, false, false>), g, b, 0, st, d, vb, cv, ky);
   }
 }
 

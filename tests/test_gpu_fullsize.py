@@ -22,11 +22,11 @@ from implicit_dpd_amd import configs
 pytestmark = pytest.mark.gpu
 
 FORCE_TOL = 1e-5
-# Below one unit of force (kT / rc) the bar is absolute: a vapour bead whose
-# only neighbours sit near a cutoff has sum |F_ij| ~ 0.1, and the FP32
-# rounding of its positions (~1e-7) times the conservative slope (|A| up to
-# 60) already moves its force by a few 1e-6.
-FORCE_FLOOR = 1.0
+# The bar is relative to the bead's sum of pair-term magnitudes down to 1e-3
+# (the dense systems' floor): a vapour bead whose only neighbours sit next to
+# the cutoff, where the FP32 error of r (~1e-7) times sigma' |xi| would miss
+# it, has those pairs' 1 - r taken from FP64 by the force pass.
+FORCE_FLOOR = 1e-3
 
 
 def _sorted_pairs(i, j):
