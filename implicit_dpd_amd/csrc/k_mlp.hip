@@ -2,12 +2,14 @@
 // hidden activations (no counterpart in the reference: it proposes MDPD
 // parameters (A, B, rd, kBT) for a target liquid density and pressure).
 //
-// Every matrix product runs on the matrix cores as
-// v_mfma_f32_32x32x8_bf16: BF16 operands, FP32 accumulation.  Operand
-// layout of that instruction for lane l (r = l % 32, h = l / 32):
-//   A (32 x 8):  A[r][4h + 0..3]
-//   B (8 x 32):  B[4h + 0..3][r]
-//   D (32 x 32): D[8(q/4) + 4h + q%4][r] in accumulator q = 0..15
+// Every matrix product runs on the matrix cores with BF16 operands and
+// FP32 accumulation: gfx950's v_mfma_f32_32x32x16_bf16 (K = 16 per
+// instruction) for the products over the 128 hidden features and over the
+// batch, v_mfma_f32_32x32x8_bf16 for the two whose K is 4 or 2 (layer 1 and
+// dY W3).  Operand layout for lane l (r = l % 32, h = l / 32):
+//   A (32 x 16): A[r][8h + 0..7]      (x8 form: A[r][4h + 0..3])
+//   B (16 x 32): B[8h + 0..7][r]      (x8 form: B[4h + 0..3][r])
+//   D (32 x 32): D[8(q/4) + 4h + q%4][r] in accumulator q = 0..15 (both)
 // Activations are rounded to BF16 (round to nearest even) where they enter
 // a product; biases, pre-activations, SiLU and the optimizer stay FP32.
 //
@@ -31,6 +33,7 @@
 namespace mdpd {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -42,6 +45,14 @@ __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float(
 __device__ __forceinline__ floatx16 mfma(bf16x4 a, bf16x4 b, floatx16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ floatx16 mfma16(bf16x8 a, bf16x8 b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+// eight consecutive BF16 values (16-byte aligned)
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
 __device__ __forceinline__ bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
   bf16x4 v;
@@ -143,11 +154,11 @@ __global__ void __launch_bounds__(64) k_mlp_rows(MlpDev m, const float* __restri
   floatx16 z2[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) z2[nt] = floatx16{};
-  for (int ks = 0; ks < kH / 8; ++ks) {
-    const bf16x4 a = load4(act + r * kH + 8 * ks + 4 * h);
+  for (int ks = 0; ks < kH / 16; ++ks) {
+    const bf16x8 a = load8(act + r * kH + 16 * ks + 8 * h);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-      z2[nt] = mfma(a, load4(m.W2b + (size_t)(32 * nt + r) * kH + 8 * ks + 4 * h), z2[nt]);
+      z2[nt] = mfma16(a, load8(m.W2b + (size_t)(32 * nt + r) * kH + 16 * ks + 8 * h), z2[nt]);
   }
   __syncthreads();
 #pragma unroll
@@ -165,10 +176,10 @@ __global__ void __launch_bounds__(64) k_mlp_rows(MlpDev m, const float* __restri
 
   // ---- layer 3: Y = H2 W3^T + b3 (N = 2 padded to 32)
   floatx16 yo = {};
-  for (int ks = 0; ks < kH / 8; ++ks) {
-    const bf16x4 a = load4(act + r * kH + 8 * ks + 4 * h);
-    const bf16x4 b = r < kOut ? load4(m.W3b + (size_t)r * kH + 8 * ks + 4 * h) : pack4(0, 0, 0, 0);
-    yo = mfma(a, b, yo);
+  for (int ks = 0; ks < kH / 16; ++ks) {
+    const bf16x8 a = load8(act + r * kH + 16 * ks + 8 * h);
+    const bf16x8 b = r < kOut ? load8(m.W3b + (size_t)r * kH + 16 * ks + 8 * h) : zero8();
+    yo = mfma16(a, b, yo);
   }
   float lsum = 0.f;
   if (r < kOut) {
@@ -218,11 +229,11 @@ __global__ void __launch_bounds__(64) k_mlp_rows(MlpDev m, const float* __restri
   floatx16 d1[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) d1[nt] = floatx16{};
-  for (int ks = 0; ks < kH / 8; ++ks) {
-    const bf16x4 a = load4(act + r * kH + 8 * ks + 4 * h);
+  for (int ks = 0; ks < kH / 16; ++ks) {
+    const bf16x8 a = load8(act + r * kH + 16 * ks + 8 * h);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-      d1[nt] = mfma(a, load4(m.W2Tb + (size_t)(32 * nt + r) * kH + 8 * ks + 4 * h), d1[nt]);
+      d1[nt] = mfma16(a, load8(m.W2Tb + (size_t)(32 * nt + r) * kH + 16 * ks + 8 * h), d1[nt]);
   }
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
@@ -234,18 +245,18 @@ __global__ void __launch_bounds__(64) k_mlp_rows(MlpDev m, const float* __restri
 
 // dW[mrow][kcol] = sum_s G[mrow][s] * H[kcol][s] over the padded batch, for
 // one 32 x 32 tile; rows >= mvalid / cols >= kvalid read as zero.
-__global__ void __launch_bounds__(64) k_mlp_wgrad(MlpDev m, uint32_t nsteps8) {
+__global__ void __launch_bounds__(64) k_mlp_wgrad(MlpDev m, uint32_t nsteps16) {
   const int l = threadIdx.x, r = l & 31, h = l >> 5;
   const MlpTile tl = m.tiles[blockIdx.x];
   const uint32_t npad = m.npad;
   const bool arow = tl.m0 + r < tl.mvalid, bcol = tl.k0 + r < tl.kvalid;
-  const uint16_t* ga = tl.G + (size_t)(tl.m0 + r) * npad + 4 * h;
-  const uint16_t* hb = tl.H + (size_t)(tl.k0 + r) * npad + 4 * h;
+  const uint16_t* ga = tl.G + (size_t)(tl.m0 + r) * npad + 8 * h;
+  const uint16_t* hb = tl.H + (size_t)(tl.k0 + r) * npad + 8 * h;
   floatx16 c = {};
-  for (uint32_t st = 0; st < nsteps8; ++st) {
-    const bf16x4 a = arow ? load4(ga + 8 * st) : pack4(0, 0, 0, 0);
-    const bf16x4 b = bcol ? load4(hb + 8 * st) : pack4(0, 0, 0, 0);
-    c = mfma(a, b, c);
+  for (uint32_t st = 0; st < nsteps16; ++st) {
+    const bf16x8 a = arow ? load8(ga + 16 * st) : zero8();
+    const bf16x8 b = bcol ? load8(hb + 16 * st) : zero8();
+    c = mfma16(a, b, c);
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -309,7 +320,7 @@ void launch_mlp_rows(const MlpDev& m, const float* x, const float* t, uint32_t n
 
 void launch_mlp_grads(const MlpDev& m, uint32_t n, uint32_t ntiles, hipStream_t st) {
   const uint32_t npad32 = div_up(n, 32) * 32;
-  hipLaunchKernelGGL(k_mlp_wgrad, dim3(ntiles), dim3(64), 0, st, m, npad32 / 8);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(ntiles), dim3(64), 0, st, m, npad32 / 16);
   hipLaunchKernelGGL(k_mlp_bgrad, dim3(2), dim3(256), 0, st, m, npad32);
 }
 
