@@ -10,6 +10,28 @@ constexpr int kWave = 64;            // CDNA wavefront
 constexpr uint32_t kCodeShift = 26;  // pair-list entry: j | stencil code << 26 (2 bits per axis)
 constexpr uint32_t kIndexMask = (1u << kCodeShift) - 1u;
 
+// x of the FP32 copies (spos.x, sb.pos.x, as raw bits): a fixed-point
+// coordinate in units of h_x / 2^24, the cell index in the bits from 24 up
+// (modulo 2^32), so a pair's x displacement is one exact integer difference
+// whichever of the three x-neighbour cells the partner is in.  Across the
+// periodic x boundary the difference is corrected by the box in those
+// units, nx << 24 (modulo 2^32).  Pair-list codes carry that correction in
+// their x field: 0 = + (nx << 24), 1 = none, 2 = - (nx << 24); their y and
+// z fields are the neighbour cell's offset -1, 0, +1 (as 0, 1, 2).
+constexpr double kFixUnits = 16777216.0;                  // 2^24 units per cell
+constexpr float kFixStep = 5.9604644775390625e-8f;        // 2^-24
+__device__ __forceinline__ uint32_t fix_x(double l, uint32_t cx, double inv_h) {
+  return (cx << 24) + (uint32_t)(int32_t)rint(l * inv_h * kFixUnits);
+}
+// x displacement of the pair (xi + off) - xj in length units (hs = h_x 2^-24)
+__device__ __forceinline__ float fix_dx(uint32_t xi_off, uint32_t xj, float hs) {
+  return (float)(int32_t)(xi_off - xj) * hs;
+}
+// the x correction of a pair-list code's x field (0, 1, 2)
+__device__ __forceinline__ uint32_t fix_wrap(uint32_t field, uint32_t wx) {
+  return field == 0u ? wx : (field == 2u ? 0u - wx : 0u);
+}
+
 // error flags (device word, sticky until read)
 enum : uint32_t {
   FLAG_NLIST_OVERFLOW = 1u,

@@ -65,12 +65,14 @@ __global__ void __launch_bounds__(256) k_rdf(DevState d, const double* __restric
         for (int ox = -1; ox <= 1; ++ox) {
           const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
           const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-          const float bx = pi.x - (float)ox * R.hf[0], by = pi.y - (float)oy * R.hf[1],
+          const uint32_t Xp = __float_as_uint(pi.x) +
+                              fix_wrap(cx + ox < 0 ? 0u : (cx + ox >= nx ? 2u : 1u), (uint32_t)nx << 24);
+          const float hs = R.hf[0] * kFixStep, by = pi.y - (float)oy * R.hf[1],
                       bz = pi.z - (float)oz * R.hf[2];
           for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) {
             if (t == s) continue;
             const float4 pj = d.spos[t];
-            const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
+            const float dx = fix_dx(Xp, __float_as_uint(pj.x), hs), dy = by - pj.y, dz = bz - pj.z;
             if (dx * dx + dy * dy + dz * dz >= cut2f) continue;
             double v[3];
 #pragma unroll

@@ -102,11 +102,11 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
   const int cz = (int)(c / (uint32_t)(nx * ny));
   const uint32_t W = (uint32_t)d.Wv;
   uint4* lst = reinterpret_cast<uint4*>(d.vlist) + (size_t)(s >> 6) * (W >> 2) * 64u + (s & 63u);
-  // the bead's x offset term for the left, own and right neighbour cell
-  const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
+  // the bead's fixed-point x and the periodic correction of x
+  const uint32_t Xi = __float_as_uint(pi.x), Wx = (uint32_t)nx << 24;
+  const float hs = hx * kFixStep;
   uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0;
   const bool wrap = !(cx >= 1 && cx <= nx - 2);
-  const uint32_t kInf = 0xFFFFFFFFu;
   for (int oz = -1; oz <= 1; ++oz) {
     int qz = cz + oz;
     qz += (qz < 0) ? nz : 0;
@@ -119,25 +119,25 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
       const float by = pi.y - (float)oy * hy;
       const uint32_t* S = d.cell_start + cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      const uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
       for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
-        uint32_t ta, tb, b1, b2;
-        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
+        // run [ta, tb) over the row's three cells; the piece across the
+        // periodic x boundary (if any) takes the box correction
+        uint32_t ta, tb, fx = 1u;
+        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; }
         else if (cx == 0) {
-          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; b1 = kInf; b2 = kInf; }
-          else { ta = S[0]; tb = S[2]; b1 = 0; b2 = S[1]; }
+          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; fx = 0u; }
+          else { ta = S[0]; tb = S[2]; }
         } else {
-          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
-          else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
+          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; }
+          else { ta = S[0]; tb = S[1]; fx = 2u; }
         }
+        const uint32_t Xp = Xi + fix_wrap(fx, Wx), C = cyz | (fx << kCodeShift);
         for (uint32_t t = ta; t < tb; ++t) {
           const float4 pj = d.spos[t];
-          const bool g2 = t >= b2, g1 = t >= b1;
-          const float bx = g2 ? X2 : (g1 ? X1 : X0);
-          const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
+          const float dx = fix_dx(Xp, __float_as_uint(pj.x), hs), dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (r2 >= cut_hi || t == s) continue;
-          const uint32_t e = t | (g2 ? C2 : (g1 ? C1 : C0));
+          const uint32_t e = t | C;
           if ((cnt & 3u) == 3u) {
             if (cnt < W) lst[(size_t)(cnt >> 2) * 64u] = make_uint4(e0, e1, e2, e);
           } else {
@@ -245,9 +245,10 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
     }
     qe = qi;
   };
-  const float X0 = pi.x + hx, X1 = pi.x, X2 = pi.x - hx;
+  // the bead's fixed-point x and the periodic correction of x
+  const uint32_t Xi = __float_as_uint(pi.x), Wx = (uint32_t)nx << 24;
+  const float hs = hx * kFixStep;
   const bool wrap = !(cx >= 1 && cx <= nx - 2);
-  const uint32_t kInf = 0xFFFFFFFFu;
   for (int oz = -1; oz <= 1; ++oz) {
     int qz = cz + oz;
     qz += (qz < 0) ? nz : 0;
@@ -261,20 +262,19 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
       const uint32_t rowq = cell_off + (uint32_t)((qz * ny + qy) * nx);
       const uint32_t* S = d.cell_start + rowq;
       const uint32_t cyz = (uint32_t)(((oy + 1) << 2) | ((oz + 1) << 4)) << kCodeShift;
-      uint32_t C0 = cyz, C1 = cyz | (1u << kCodeShift), C2 = cyz | (2u << kCodeShift);
-      asm volatile("" : "+v"(C0), "+v"(C1), "+v"(C2));
       for (int pc = 0; pc < (wrap ? 2 : 1); ++pc) {
-        // run [ta, tb) over the row's three cells; slots >= b2 are in the
-        // right cell, >= b1 the own cell
-        uint32_t ta, tb, b1, b2;
-        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; b1 = S[cx]; b2 = S[cx + 1]; }
+        // run [ta, tb) over the row's three cells; the piece across the
+        // periodic x boundary (if any) takes the box correction
+        uint32_t ta, tb, fx = 1u;
+        if (!wrap) { ta = S[cx - 1]; tb = S[cx + 2]; }
         else if (cx == 0) {
-          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; b1 = kInf; b2 = kInf; }
-          else { ta = S[0]; tb = S[2]; b1 = 0; b2 = S[1]; }
+          if (pc == 0) { ta = S[nx - 1]; tb = S[nx]; fx = 0u; }
+          else { ta = S[0]; tb = S[2]; }
         } else {
-          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; b1 = S[nx - 1]; b2 = kInf; }
-          else { ta = S[0]; tb = S[1]; b1 = 0; b2 = 0; }
+          if (pc == 0) { ta = S[nx - 2]; tb = S[nx]; }
+          else { ta = S[0]; tb = S[1]; fx = 2u; }
         }
+        const uint32_t Xp = Xi + fix_wrap(fx, Wx), C = cyz | (fx << kCodeShift);
         // two candidates per iteration, two gathers in flight per wave; the
         // second of an odd run's last pair is a repeated load, not queued.
         // The queue keeps room for both (drained when fewer than two are free).
@@ -282,21 +282,17 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
           const uint32_t u = min(t + 1u, tb - 1u);
           const float4 pa = ld_slot(d.spos, t), pb = ld_slot(d.spos, u);
           {
-            const bool g2 = t >= b2, g1 = t >= b1;
-            const float bx = g2 ? X2 : (g1 ? X1 : X0);
-            const float dx = bx - pa.x, dy = by - pa.y, dz = bz - pa.z;
+            const float dx = fix_dx(Xp, __float_as_uint(pa.x), hs), dy = by - pa.y, dz = bz - pa.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             // written always (the queue has room for two more), kept only
             // in range: no exec-mask branch per candidate
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | C, __float_as_uint(r2));
             qe += r2 < cut_hi ? qs : 0u;
           }
           {
-            const bool g2 = u >= b2, g1 = u >= b1;
-            const float bx = g2 ? X2 : (g1 ? X1 : X0);
-            const float dx = bx - pb.x, dy = by - pb.y, dz = bz - pb.z;
+            const float dx = fix_dx(Xp, __float_as_uint(pb.x), hs), dy = by - pb.y, dz = bz - pb.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | (g2 ? C2 : (g1 ? C1 : C0)), __float_as_uint(r2));
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | C, __float_as_uint(r2));
             qe += (r2 < cut_hi && u != t) ? qs : 0u;
           }
           if (__ballot(qe > qhi) != 0ull) drain();
@@ -363,6 +359,8 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
   const float rd2f = R.rd2f, inv_rd = R.inv_rd, crho = R.crho;
   const float cut_hi = fmaxf(1.0f, rd2f) + kGuard;
   const float4 pi = d.spos[s];
+  const uint32_t Xi = __float_as_uint(pi.x), Wx = (uint32_t)R.nc[0] << 24;
+  const float hs = hx * kFixStep;
   const uint32_t W = (uint32_t)d.W, Wv = (uint32_t)d.Wv;
   uint4* lst = const_cast<uint4*>(tile_list(d, s));
   const uint4* vl = reinterpret_cast<const uint4*>(d.vlist) + (size_t)(s >> 6) * (Wv >> 2) * 64u +
@@ -377,7 +375,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
     const float4 pj = d.spos[t];
-    const float dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+    const float dx = fix_dx(Xi + fix_wrap(code & 3u, Wx), __float_as_uint(pj.x), hs);
     const float dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
     const float dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -517,15 +515,16 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   // single system: the coefficients come from a block-shared table indexed
   // by the type pair instead of per-lane selects
   __shared__ float4 ptab[18];
-  // single system: the image offset o * h of each stencil code (o in
-  // {-1, 0, 1} per axis, exact in FP32), so pi - o h rounds once as before
+  // single system: per stencil code, the periodic x correction of the
+  // fixed-point x (as raw bits) and the y, z image offsets o h (o in
+  // {-1, 0, 1}, exact in FP32), so pi - o h rounds once
   __shared__ float4 otab[64];
   if (!MULTI) {
     if (NS > 1) fill_pair_table(d.reps[0], ptab);
     if (threadIdx.x < 64u) {
       const uint32_t k = threadIdx.x;
       const Replica& R0 = d.reps[0];
-      otab[k] = make_float4((float)((int)(k & 3u) - 1) * R0.hf[0],
+      otab[k] = make_float4(__uint_as_float(fix_wrap(k & 3u, (uint32_t)R0.nc[0] << 24)),
                             (float)((int)((k >> 2) & 3u) - 1) * R0.hf[1],
                             (float)((int)(k >> 4) - 1) * R0.hf[2], 0.0f);
     }
@@ -559,6 +558,8 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     for (int k = 0; k < ND; ++k) MB[k][q] = R.mb[ti][q][k];
   }
   const float hx = R.hf[0], hy = R.hf[1], hz = R.hf[2];
+  const uint32_t Xi = __float_as_uint(pi.x), Wx = (uint32_t)R.nc[0] << 24;
+  const float hs = hx * kFixStep;
   const float arc = R.arc, inv_rdm = R.inv_rdm;
   const bool rdm1 = R.rdm_is_one != 0;
   const bool has_c1 = R.has_c1 != 0;
@@ -579,11 +580,11 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     float dx, dy, dz;
     if (!MULTI) {
       const float4 o = otab[code];
-      dx = (pi.x - o.x) - pj.x;
+      dx = fix_dx(Xi + __float_as_uint(o.x), __float_as_uint(pj.x), hs);
       dy = (pi.y - o.y) - pj.y;
       dz = (pi.z - o.z) - pj.z;
     } else {
-      dx = (pi.x - (float)((int)(code & 3u) - 1) * hx) - pj.x;
+      dx = fix_dx(Xi + fix_wrap(code & 3u, Wx), __float_as_uint(pj.x), hs);
       dy = (pi.y - (float)((int)((code >> 2) & 3u) - 1) * hy) - pj.y;
       dz = (pi.z - (float)((int)(code >> 4) - 1) * hz) - pj.z;
     }
@@ -787,12 +788,14 @@ __global__ void __launch_bounds__(256) k_pairs_list(DevState d, float cut2f, dou
       for (int ox = -1; ox <= 1; ++ox) {
         const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
         const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-        const float bx = pi.x - (float)ox * R.hf[0], by = pi.y - (float)oy * R.hf[1],
+        const uint32_t Xp = __float_as_uint(pi.x) +
+                            fix_wrap(cx + ox < 0 ? 0u : (cx + ox >= nx ? 2u : 1u), (uint32_t)nx << 24);
+        const float hs = R.hf[0] * kFixStep, by = pi.y - (float)oy * R.hf[1],
                     bz = pi.z - (float)oz * R.hf[2];
         for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) {
           if (t == s) continue;
           const float4 pj = d.spos[t];
-          const float dx = bx - pj.x, dy = by - pj.y, dz = bz - pj.z;
+          const float dx = fix_dx(Xp, __float_as_uint(pj.x), hs), dy = by - pj.y, dz = bz - pj.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (r2 >= cut2f + kGuard) continue;
           if (decide(r2, cut2f, cut2, d, R, s, t)) {

@@ -318,6 +318,7 @@ __global__ void __launch_bounds__(256) k_permute(DevState d, int predicted, int 
     // skin mode: the displacement test of the coming steps starts here
     if (skin) d.dref[a * cap + s] = dl;
     l[a] = (float)(x - (double)cc[a] * R.h[a]);
+    if (a == 0) l[0] = __uint_as_float(fix_x(x - (double)cc[0] * R.h[0], cc[0], R.inv_h[0]));
     if (full) {
       d.alt.vel[a * cap + s] = d.vel[a * cap + i];
       d.alt.acc[a * cap + s] = d.acc[a * cap + i];
@@ -395,6 +396,7 @@ __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
     bx[a] = R.box[a];
     h[a] = R.h[a];
   }
+  const double ihx = R.inv_h[0];
   const uint32_t c = key - R.cell_off;
   const uint32_t nx = (uint32_t)R.nc[0], ny = (uint32_t)R.nc[1];
   const uint32_t cc[3] = {c % nx, (c / nx) % ny, c / (nx * ny)};
@@ -409,7 +411,8 @@ __global__ void __launch_bounds__(256) k_step_permute(DevState d) {
     st_hint<ST>(&d.alt.pos[a * cap + t], o.p);
     d.alt.vbuf[a * cap + t] = o.vb;
     st_hint<ST>(&d.alt.delta[a * cap + t], o.dl);
-    l[a] = (float)(o.p - (double)cc[a] * h[a]);
+    l[a] = a == 0 ? __uint_as_float(fix_x(o.p - (double)cc[0] * h[0], cc[0], ihx))
+                  : (float)(o.p - (double)cc[a] * h[a]);
     v[a] = (float)o.vb;
   }
   st_hint<ST>(&d.alt.id[t], id);
@@ -779,7 +782,7 @@ __global__ void __launch_bounds__(256) k_drift_skin(DevState d) {
     double x = p[a] - (double)cc[a] * R.h[a];
     if (x > 0.5 * R.box[a]) x -= R.box[a];
     else if (x < -0.5 * R.box[a]) x += R.box[a];
-    l[a] = (float)x;
+    l[a] = a == 0 ? __uint_as_float(fix_x(x, cc[0], R.inv_h[0])) : (float)x;
     v[a] = (float)vb;
   }
   if (!fin) atomicOr(d.flags, FLAG_NONFINITE);
