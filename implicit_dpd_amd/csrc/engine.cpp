@@ -294,7 +294,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.spos, cap64);
   ALLOC(d.sb, cap64);
   ALLOC(d.scell, cap64);
-  ALLOC(d.sdens, (size_t)c->nd * cap64);
+  ALLOC(d.sdens, (size_t)(c->nd == 1 ? 1 : 4) * cap64);  // one float, or float4 per bead
   ALLOC(d.nlist, (size_t)cap64 * c->tune.nlist_width);
   ALLOC(d.ncount, cap64);
   ALLOC(d.reps, n_rep);
@@ -751,11 +751,14 @@ int mdpd_download_soa(mdpd_ctx* c, uint32_t n, double* pos, double* vel, double*
   if ((rc = getu(d.unit, unit))) return rc;
   if (dens) {
     memset(dens, 0, sizeof(double) * 3 * (size_t)n);
-    if (c->have_density)
-      for (int k = 0; k < c->nd; ++k) {
-        HIPCHK(c, hipMemcpy(f.data(), d.sdens + (size_t)k * d.cap, 4 * (size_t)n, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; ++i) dens[3 * (size_t)orig[i] + k] = f[i];
-      }
+    if (c->have_density) {
+      // one float per bead, or (rho_0, rho_1, rho_2, 0) per bead
+      const size_t per = c->nd == 1 ? 1 : 4;
+      std::vector<float> g(per * (size_t)n);
+      HIPCHK(c, hipMemcpy(g.data(), d.sdens, 4 * per * (size_t)n, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < c->nd; ++k) dens[3 * (size_t)orig[i] + k] = g[per * i + k];
+    }
   }
   return MDPD_OK;
 }

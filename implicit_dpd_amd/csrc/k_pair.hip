@@ -73,6 +73,15 @@ __device__ __forceinline__ float4 ld_slot(const float4* __restrict__ base, uint3
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (t << 4));
 }
 
+// the bead's densities: one float with one species, else (rho_0, rho_1,
+// rho_2, 0) as one 16-byte record, so the force pass gathers a neighbour's
+// three densities with one load
+template <int ND>
+__device__ __forceinline__ void store_dens(const DevState& d, uint32_t s, const float (&rho)[ND]) {
+  if (ND == 1) d.sdens[s] = rho[0];
+  else reinterpret_cast<float4*>(d.sdens)[s] = make_float4(rho[0], rho[1 % ND], rho[2 % ND], 0.f);
+}
+
 __device__ __forceinline__ const uint4* tile_list(const DevState& d, uint32_t s) {
   return reinterpret_cast<const uint4*>(d.nlist) + (size_t)(s >> 6) * ((uint32_t)d.W >> 2) * 64u +
          (s & 63u);
@@ -173,6 +182,16 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
 // sums and the pair list are those of a plain per-candidate loop.
 constexpr uint32_t kDensQ = 8;
 
+// The queued r^2 word.  A kept candidate has r^2 < 1 + kGuard < 2 (rd <= 1),
+// so the float's two top bits (sign and the exponent's top bit) are zero;
+// with several density species they carry the candidate's type (.w & 3 of
+// its FP32 copy), which the drain then needs no gather for.
+template <int ND>
+__device__ __forceinline__ uint32_t queue_r2(float r2, float w) {
+  if (ND == 1) return __float_as_uint(r2);
+  return __float_as_uint(r2) | (__float_as_uint(w) << 30);
+}
+
 template <int ND, bool MULTI, bool RDLE1, int BS>
 __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
   __shared__ uint2 dens_queue[kDensQ * BS];  // [kDensQ][BS]
@@ -209,7 +228,7 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
     for (uint32_t k = qi; k < qe; k += qs) {
       const uint2 en = *reinterpret_cast<const uint2*>(qbase + k);
       const uint32_t e = en.x, t = e & kIndexMask;
-      float r2 = __uint_as_float(en.y);
+      float r2 = __uint_as_float(ND == 1 ? en.y : (en.y & 0x3FFFFFFFu));
       // outside the guard bands the FP32 r^2 decides; the clamp of w at 0
       // is then the rd test, and with RDLE1 no cutoff compare is needed
       bool in_rc = RDLE1 ? true : (r2 < 1.0f);
@@ -226,7 +245,7 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
       const float v = crho * w * w;
       if (ND == 1) rho[0] += v;
       else {
-        const uint32_t tj = __float_as_uint(d.spos[t].w) & 3u;
+        const uint32_t tj = en.y >> 30;  // the candidate's type, from the scan
 #pragma unroll
         for (int k2 = 0; k2 < ND; ++k2) rho[k2] += (tj == (uint32_t)k2) ? v : 0.f;
       }
@@ -286,13 +305,13 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
             const float r2 = dx * dx + dy * dy + dz * dz;
             // written always (the queue has room for two more), kept only
             // in range: no exec-mask branch per candidate
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | C, __float_as_uint(r2));
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | C, queue_r2<ND>(r2, pa.w));
             qe += r2 < cut_hi ? qs : 0u;
           }
           {
             const float dx = fix_dx(Xp, __float_as_uint(pb.x), hs), dy = by - pb.y, dz = bz - pb.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | C, __float_as_uint(r2));
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | C, queue_r2<ND>(r2, pb.w));
             qe += (r2 < cut_hi && u != t) ? qs : 0u;
           }
           if (__ballot(qe > qhi) != 0ull) drain();
@@ -308,8 +327,7 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
                                 : make_uint4(e0, e1, e2, 0u);
     lst[(size_t)(cnt >> 2) * 64u] = ch;
   }
-#pragma unroll
-  for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+  store_dens<ND>(d, s, rho);
   if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
   d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
@@ -422,8 +440,7 @@ __global__ void __launch_bounds__(256) k_density_skin(DevState d) {
                                 : make_uint4(e0, e1, e2, 0u);
     lst[(size_t)(cnt >> 2) * 64u] = ch;
   }
-#pragma unroll
-  for (int k = 0; k < ND; ++k) d.sdens[(size_t)k * d.cap + s] = rho[k];
+  store_dens<ND>(d, s, rho);
   if (ND == 1) reinterpret_cast<float*>(&d.sb[s].vel)[3] = rho[0];
   d.ncount[s] = cnt;
   if (cnt > W) atomicOr(d.flags, FLAG_NLIST_OVERFLOW);
@@ -545,8 +562,14 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   const uint32_t wi = __float_as_uint(pi.w);
   const uint32_t idi = wi >> 2, ti = NS == 1 ? 0u : (wi & 3u), ti3 = 3u * ti;
   float rho_i[ND];
-#pragma unroll
-  for (int k = 0; k < ND; ++k) rho_i[k] = d.sdens[(size_t)k * d.cap + s];
+  if (ND == 1) {
+    rho_i[0] = d.sdens[s];
+  } else {
+    const float4 ri = reinterpret_cast<const float4*>(d.sdens)[s];
+    rho_i[0] = ri.x;
+    rho_i[1 % ND] = ri.y;
+    rho_i[2 % ND] = ri.z;
+  }
   float A0[NS], C1[NS], SG[NS], GM[NS], MB[ND][NS];
 #pragma unroll
   for (int q = 0; q < NS; ++q) {
@@ -644,8 +667,12 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     if (ND == 1) {
       mbs = mbk[0] * (rho_i[0] + vj.w);
     } else {
+      {
+        const float4 dj = reinterpret_cast<const float4*>(d.sdens)[t];
+        const float rj[3] = {dj.x, dj.y, dj.z};
 #pragma unroll
-      for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + d.sdens[(size_t)k * d.cap + t]);
+        for (int k = 0; k < ND; ++k) mbs += mbk[k] * (rho_i[k] + rj[k]);
+      }
     }
     // with rdm = 1, 1 - r * 1 is 1 - r bit for bit, so only the reference
     // envelope (whose wc may come from FP64 near the cutoff) needs wc itself

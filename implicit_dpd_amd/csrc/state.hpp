@@ -56,7 +56,7 @@ struct DevState : Master {
   float4* spos;         // [cap] the positions alone, for the cell and list walks
   SBead* sb;            // [cap] position + velocity, for the force pass
   uint32_t* scell;      // global cell of each slot at the last rebuild
-  float* sdens;         // [nd][cap] density
+  float* sdens;         // [cap] density (one species), else float4 [cap] (rho_0, rho_1, rho_2, 0)
   uint32_t* nlist;      // wave-tiled pair list [cap/64][W/4][64][4]
   uint32_t* ncount;     // [cap]
   int32_t W;
