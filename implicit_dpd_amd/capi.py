@@ -217,6 +217,7 @@ PROTOTYPES = {
     "mdpd_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int32, P(C.c_double)]),
     "mdpd_read_run_param": (C.c_int, [C.c_char_p, P(MdpdParams), P(MdpdRunInput)]),
     "mdpd_run_timed": (C.c_int, [VP, C.c_uint32, _DP]),
+    "mdpd_prepare_run": (C.c_int, [VP, C.c_uint32]),
     # initial configurations
     "mdpd_read_config_param": (C.c_int, [C.c_char_p, P(MdpdConfigSpec)]),
     "mdpd_config_count": (C.c_int, [P(MdpdConfigSpec), _U32P]),

@@ -76,6 +76,7 @@ int fail(mdpd_ctx* c, int code, const std::string& msg);
 void invalidate_graphs(mdpd_ctx* c);
 void swap_master(mdpd_ctx* c);
 int run_steps(mdpd_ctx* c, uint32_t nsteps);
+int prepare_graphs(mdpd_ctx* c, uint32_t nsteps);
 int check_flags(mdpd_ctx* c);
 // device address of the fixed scratch of mdpd_rdf_noalloc on device `dev`
 // (looked up once per device, at context creation)

@@ -459,11 +459,73 @@ int mdpd::check_flags(mdpd_ctx* c) {
   return MDPD_OK;
 }
 
+namespace {
+// graph index of a step variant
+int step_variant(bool virial, bool cm, bool first, bool chem) {
+  return (virial ? 1 : 0) | (cm ? 2 : 0) | (first ? 4 : 0) | (chem ? 8 : 0);
+}
+
+// records the graph of step variant v for master-set parity p without
+// running it; the host's parity is left as it was
+int capture_variant(mdpd_ctx* c, int p, int v) {
+  const int p0 = c->parity;
+  if (c->parity != p) swap_master(c);
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  enqueue_step(c, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0, (v & 8) != 0);  // flips the parity
+  HIPCHK(c, hipStreamEndCapture(c->st, &g));
+  const hipError_t e = hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (c->parity != p0) swap_master(c);
+  if (e != hipSuccess) {
+    c->gexec[p][v] = nullptr;
+    return fail(c, MDPD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return MDPD_OK;
+}
+}  // namespace
+
+// Captures, before anything is launched, the graph of every step variant
+// (parity of the master set x virial / CM / key-pass / chemistry step) that
+// the next nsteps steps of run_steps use, so that no capture falls between
+// the launches of a run.  Skin mode and eager or profiled runs capture
+// nothing.
+int mdpd::prepare_graphs(mdpd_ctx* c, uint32_t nsteps) {
+  if (!c->tune.use_graph || c->profiling || c->tune.verlet_skin > 0.0f || nsteps == 0)
+    return MDPD_OK;
+  // step k of the run: time c->time + k, parity c->parity ^ (k & 1), a key
+  // pass when k = 0 or step k - 1 was a CM step, the virial when k is the
+  // last.  Past the chemistry's start and two CM periods the kinds repeat
+  // (period lcm(2, cm_corr)), so those steps and the last cover every
+  // variant of the run.
+  const uint32_t cmc = (uint32_t)c->cm_corr;
+  uint32_t span = 2u * cmc + 2u;
+  if (c->chem_on && c->beg_chem > c->time) span += c->beg_chem - c->time;
+  auto visit = [&](uint32_t k) -> int {
+    const uint32_t t = c->time + k;
+    const int p = c->parity ^ (int)(k & 1u);
+    const bool first = k == 0u || ((t - 1u) % cmc) == 0u;
+    const int v = step_variant(k + 1u == nsteps, (t % cmc) == 0u, first,
+                               c->chem_on && t >= c->beg_chem);
+    return c->gexec[p][v] ? MDPD_OK : capture_variant(c, p, v);
+  };
+  for (uint32_t k = 0; k < nsteps && k <= span; ++k) {
+    const int rc = visit(k);
+    if (rc) return rc;
+  }
+  if (nsteps - 1u > span) return visit(nsteps - 1u);
+  return MDPD_OK;
+}
+
 int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
   const bool skin = c->tune.verlet_skin > 0.0f;
   if (c->chem_on && skin)
     return fail(c, MDPD_ERR_UNSUPPORTED,
                 "chemistry inside mdpd_run needs the every-step cell list (verlet_skin = 0)");
+  {
+    const int rc = prepare_graphs(c, nsteps);
+    if (rc) return rc;
+  }
   const uint32_t time0 = c->time;
   // skin mode: the first step builds when the host has moved or reordered
   // beads since the last build
@@ -478,7 +540,7 @@ int mdpd::run_steps(mdpd_ctx* c, uint32_t nsteps) {
     const bool cm = (c->time % c->cm_corr) == 0;
     const bool chem = c->chem_on && c->time >= c->beg_chem;
     if (c->tune.use_graph && !c->profiling) {
-      const int v = (virial ? 1 : 0) | (cm ? 2 : 0) | (first && !skin ? 4 : 0) | (chem ? 8 : 0);
+      const int v = step_variant(virial, cm, first && !skin, chem);
       // (a skin-mode step keeps the storage where it is: no parity flip)
       const int p = c->parity;
       if (!c->gexec[p][v]) {
@@ -1066,6 +1128,13 @@ int mdpd_run(mdpd_ctx* c, uint32_t nsteps) {
   return check_flags(c);
 }
 
+int mdpd_prepare_run(mdpd_ctx* c, uint32_t nsteps) {
+  if (!c) return MDPD_ERR_ARG;
+  if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
+  if (c->drifted) return fail(c, MDPD_ERR_STATE, "prepare_run called between drift and kick");
+  return prepare_graphs(c, nsteps);
+}
+
 int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
   if (!c || !gpu_ms) return MDPD_ERR_ARG;
   if (!c->d.n) return fail(c, MDPD_ERR_STATE, "no beads uploaded");
@@ -1073,6 +1142,11 @@ int mdpd_run_timed(mdpd_ctx* c, uint32_t nsteps, double* gpu_ms) {
     return fail(c, MDPD_ERR_STATE, "the chemistry stopped the run (max_amp_num reached); "
                                    "mdpd_set_chemistry starts it again");
   if (c->drifted) return fail(c, MDPD_ERR_STATE, "run called between drift and kick");
+  {
+    // the step graphs are recorded before the first event
+    const int rc = prepare_graphs(c, nsteps);
+    if (rc) return rc;
+  }
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));

@@ -33,6 +33,14 @@ def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+def _nsteps(n):
+    # mdpd_run takes an unsigned count: a negative one would wrap to ~4e9 steps
+    n = int(n)
+    if n < 0:
+        raise ValueError(f"negative step count {n}")
+    return n
+
+
 def observables_dict(o):
     return {
         "kin_temp": np.array(o.kin_temp[:]),
@@ -205,12 +213,17 @@ class Engine:
         return bool(st.value), ev.value, ins.value
 
     def run(self, nsteps):
-        self._c(self.L.mdpd_run(self.h, int(nsteps)))
+        self._c(self.L.mdpd_run(self.h, _nsteps(nsteps)))
+
+    def prepare_run(self, nsteps):
+        """Records the step graphs the next run(nsteps) launches, so that
+        no graph capture falls inside a timed run."""
+        self._c(self.L.mdpd_prepare_run(self.h, _nsteps(nsteps)))
 
     def run_timed(self, nsteps):
         """Runs nsteps (captured graphs) and returns the device time in ms."""
         ms = C.c_double()
-        self._c(self.L.mdpd_run_timed(self.h, int(nsteps), C.byref(ms)))
+        self._c(self.L.mdpd_run_timed(self.h, _nsteps(nsteps), C.byref(ms)))
         return ms.value
 
     def skin_stats(self):

@@ -308,6 +308,13 @@ int mdpd_run(mdpd_ctx* ctx, uint32_t nsteps);
 /* mdpd_run with HIP events recorded on the context's stream around the
    steps; *gpu_ms is the device time between them.  Used by the benchmark. */
 int mdpd_run_timed(mdpd_ctx* ctx, uint32_t nsteps, double* gpu_ms);
+/* Records, without running anything, the step graphs that the next
+   mdpd_run(ctx, nsteps) launches (one per parity of the master storage and
+   kind of step: virial, CM-drift removal, key pass, chemistry).  mdpd_run
+   and mdpd_run_timed do this themselves before their first launch (and
+   mdpd_run_timed before its first event); a caller that times its own runs
+   calls it first.  No reference counterpart (the reference has no graphs). */
+int mdpd_prepare_run(mdpd_ctx* ctx, uint32_t nsteps);
 /* Skin mode (mdpd_tuning.verlet_skin > 0): the number of steps so far that
    rebuilt cells and skin list, and (entries may be NULL) the skin-list
    entries of the last build summed over the beads.  No reference

@@ -205,15 +205,20 @@ def test_split_runs_and_cm_steps_are_bit_identical(engine_mod):
     on a run's first step and after a CM-drift removal, which bin the beads
     in a key pass of their own: one 12-step run, the same 12 steps as runs of
     5 + 4 + 3, and the kernel-by-kernel launches all give the same
-    trajectory bit for bit, with a CM-drift removal every third step."""
+    trajectory bit for bit, with a CM-drift removal every third step; so do
+    runs whose step graphs were recorded beforehand by mdpd_prepare_run (for
+    the run's own length, and for a longer run than the one that follows)."""
     name, p0, s = SYSTEMS[2]
     p = type(p0).from_buffer_copy(p0)
     p.cm_corr = 3
     res = []
-    for graph, split in ((1, (12,)), (1, (5, 4, 3)), (0, (12,)), (0, (1, 2, 9))):
+    for graph, split, prep in ((1, (12,), 0), (1, (5, 4, 3), 0), (0, (12,), 0), (0, (1, 2, 9), 0),
+                               (1, (12,), 1), (1, (5, 4, 3), 1), (1, (7, 5), 9)):
         e = _engine(engine_mod, p, s, time=4)
         e.set_tuning(use_graph=graph)
         for k in split:
+            if prep:
+                e.prepare_run(k * prep)
             e.run(k)
         res.append(e.download())
         e.close()
