@@ -220,7 +220,7 @@ def train_surrogate(X, Y, epochs=300, seed=3):
     return net, xs, ys, hist
 
 
-def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=6,
+def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, max_runs=12,
                         reach=0.05):
     """Surrogate-guided root finding on A at fixed B, rd, kBT = 1.
 
@@ -229,7 +229,11 @@ def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, 
     is checked once at the closest A and reported unreachable.  After a miss
     the next A is a Newton step on the measured residual, with the slope of
     the network at the current A (and the secant of the last two plain runs
-    once there are two)."""
+    once there are two, when it has the network's sign).  Once plain runs lie
+    on both sides of the target (rho falls as A rises), a step that leaves
+    that bracket is replaced by its midpoint (safeguarded Newton), so noisy
+    runs cannot make it oscillate.  The run budget is the bracketing
+    sweep's."""
     grid = np.linspace(A_RANGE[0], A_RANGE[1], 4001)
     cand = np.stack([grid, np.full_like(grid, B), np.full_like(grid, rd), np.ones_like(grid)], 1)
     pred = ys.inv(net.forward(xs.fwd(cand)))[:, 0]
@@ -245,11 +249,16 @@ def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, 
         rho, _ = plain(a)
         runs.append((a, rho, float(pred[i])))
         return None, runs
+    below = above = None  # the nearest A with rho above / below the target
     for _ in range(max_runs):
         rho, _ = plain(a)
         runs.append((a, rho, float(np.interp(a, grid, pred))))
         if abs(rho - target) <= TOL * target:
             return a, runs
+        if rho > target:
+            below = a if below is None else max(below, a)
+        else:
+            above = a if above is None else min(above, a)
         sl = slope_at(a)
         if len(runs) >= 2 and abs(runs[-1][0] - runs[-2][0]) > 1e-6:
             sec = (runs[-1][1] - runs[-2][1]) / (runs[-1][0] - runs[-2][0])
@@ -258,6 +267,8 @@ def surrogate_calibrate(net, xs, ys, plain, target=TARGET_RHO, B=25.0, rd=0.75, 
         if sl == 0 or not np.isfinite(sl):
             break
         a = float(np.clip(a + (target - rho) / sl, A_RANGE[0], A_RANGE[1]))
+        if below is not None and above is not None and below < above and not below < a < above:
+            a = 0.5 * (below + above)
     return None, runs
 
 

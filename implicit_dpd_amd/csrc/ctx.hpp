@@ -60,7 +60,7 @@ struct mdpd_ctx {
   bool chem_stopped = false;  // an in-run event reached max_amp_num
   uint32_t beg_chem = 0;
   int parity = 0;            // which buffer set is current (flips every rebuild)
-  uint32_t* inv_orig = nullptr;  // scratch [cap]
+  double* dl_load = nullptr;  // observer scratch: delta_sumr in load order, [3][cap]
   bool profiling = false;
   // profiling: kPhases + 1 events per profiled step, read back lazily
   std::vector<hipEvent_t> evpool;

@@ -284,7 +284,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
   ALLOC(d.chain, cap);
   ALLOC(d.alt.chain, cap);
   ALLOC(d.inv_perm, cap);
-  ALLOC(c->inv_orig, cap);
+  ALLOC(c->dl_load, 3 * (size_t)cap);
   ALLOC(d.key, cap);
   ALLOC(d.cell_cnt, (size_t)cells + 1);
   ALLOC(d.cell_start, (size_t)cells + 1);
@@ -1241,7 +1241,7 @@ int mdpd_sync(mdpd_ctx* c) {
 static int observe_impl(mdpd_ctx* c, mdpd_observables* out, int all) {
   if (!c || !out) return MDPD_ERR_ARG;
   const DevState& d = c->d;
-  launch_observe(d, c->d_amp_num, c->inv_orig, c->obs_out, c->st);
+  launch_observe(d, c->d_amp_num, c->dl_load, c->obs_out, c->st);
   std::vector<double> q((size_t)c->n_rep * 13), bv((size_t)c->n_rep * 3);
   HIPCHK(c, hipMemcpyAsync(q.data(), c->obs_out, sizeof(double) * q.size(), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipMemcpyAsync(bv.data(), c->vir_bonded, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, c->st));

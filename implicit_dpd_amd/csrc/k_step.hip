@@ -590,14 +590,19 @@ void launch_cm_drift(const DevState& d, hipStream_t st) {
 //  reference's solvent loop, observer.hpp:171); 11 sum over amphiphiles of
 //  |mean delta of beads 4a..4a+3|^2 (observer.hpp:160-167); 12 number of
 //  pair-list entries (sorted slots of the replica).
-__global__ void __launch_bounds__(256) k_inv_orig(DevState d, uint32_t* __restrict__ inv_orig) {
+// delta_sumr in load order, dl[a][orig[i]] = delta[a][i]: the per-amphiphile
+// term then reads each amphiphile's four beads as one contiguous run
+__global__ void __launch_bounds__(256) k_delta_load_order(DevState d, double* __restrict__ dl) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < dev_n(d)) inv_orig[d.orig[i]] = i;
+  if (i >= dev_n(d)) return;
+  const uint32_t o = d.orig[i];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) dl[a * (size_t)d.cap + o] = d.delta[a * d.cap + i];
 }
 
 __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
                                                          const uint32_t* __restrict__ amp_num,
-                                                         const uint32_t* __restrict__ inv_orig,
+                                                         const double* __restrict__ dl,
                                                          double* __restrict__ part) {
 #pragma clang fp contract(off)
   const int r = blockIdx.y;
@@ -626,15 +631,15 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
     // index amp_num (observer.hpp:171), amphiphile a = beads 4a..4a+3 (:163-164)
     if (d.orig[i] - b0 >= na) acc[10].add(dd);
     if ((k & 3u) == 0 && (k >> 2) < na && k + 3 < n_r) {
-      uint32_t q[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) q[u] = inv_orig[b0 + k + u];
+      // load-order beads b0 + k .. b0 + k + 3
+      const double* q = dl + b0 + k;
       double m[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        m[a] = d.delta[a * cap + q[0]] + d.delta[a * cap + q[1]];
-        m[a] += d.delta[a * cap + q[2]];
-        m[a] += d.delta[a * cap + q[3]];
+        const double* qa = q + a * (size_t)cap;
+        m[a] = qa[0] + qa[1];
+        m[a] += qa[2];
+        m[a] += qa[3];
         m[a] /= 4.0;
       }
       acc[11].add(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
@@ -643,11 +648,11 @@ __global__ void __launch_bounds__(kRedThreads) k_observe(DevState d,
   block_reduce_write<13>(acc, part + ((size_t)r * kRedBlocks + blockIdx.x) * 2 * 13);
 }
 
-void launch_observe(const DevState& d, const uint32_t* amp_num, uint32_t* inv_orig, double* out,
+void launch_observe(const DevState& d, const uint32_t* amp_num, double* dl_load, double* out,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_inv_orig, grid1(d.nl, 256), dim3(256), 0, st, d, inv_orig);
+  hipLaunchKernelGGL(k_delta_load_order, grid1(d.nl, 256), dim3(256), 0, st, d, dl_load);
   hipLaunchKernelGGL(k_observe, dim3(kRedBlocks, d.n_rep), dim3(kRedThreads), 0, st, d, amp_num,
-                     (const uint32_t*)inv_orig, d.red);
+                     (const double*)dl_load, d.red);
   hipLaunchKernelGGL(k_red_final<13>, dim3(d.n_rep), dim3(64), 0, st, d.red, out,
                      (const uint32_t*)nullptr);
 }

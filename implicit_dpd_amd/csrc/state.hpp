@@ -181,8 +181,8 @@ void launch_copy_back_gated(const DevState& d, hipStream_t st);
 void launch_skin_build(const DevState& d, hipStream_t st);
 // also counts the build of this step (rebuild[1]) and clears rebuild[0]
 void launch_density_skin(const DevState& d, int nd, int block, hipStream_t st);
-// observables: out[n_rep][13] FP64; inv_orig scratch [cap]
-void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, uint32_t* inv_orig,
+// observables: out[n_rep][13] FP64; dl_load scratch FP64 [3][cap]
+void launch_observe(const DevState& d, const uint32_t* amp_num /*[n_rep]*/, double* dl_load,
                     double* out, hipStream_t st);
 void launch_density_profile(const DevState& d, int axis, int nbins, uint32_t* hist,
                             hipStream_t st);
