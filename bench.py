@@ -243,12 +243,19 @@ def workload(name, chains):
 RELAX_STEPS = {"membrane": 2000}
 
 
-def relax(w, Engine, steps, skin=0.0):
+def relax(w, Engine, steps, skin=0.0, state=None):
     """Runs the workload's constructed state for `steps` production steps in
     a new engine and returns that engine; the result (positions,
     velocities, accelerations, types) becomes the workload's initial state
-    for the CPU baseline."""
-    e = w.start(Engine, warmup=steps, skin=skin)
+    for the CPU baseline.  With `state`, a checkpoint file of that relaxed
+    state: loaded when it exists (no relaxation steps run), else written."""
+    if state and os.path.exists(state):
+        e = w.start(Engine, warmup=1, skin=skin)
+        e.checkpoint_load(state)
+    else:
+        e = w.start(Engine, warmup=steps, skin=skin)
+        if state:
+            e.checkpoint_save(state)
     g = e.download()
     s = dict(w.systems)
     for k in ("pos", "vel", "acc"):
@@ -449,6 +456,9 @@ def main():
                          "same model on random chains; the others are the smaller "
                          "configurations of the parity tests")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--state", default=None,
+                    help="checkpoint of the relaxed state: loaded if the file exists, else "
+                         "written after the relaxation (profiling runs share one state)")
     ap.add_argument("--relax", type=int, default=-1,
                     help="steps run from the constructed state before the warm-up "
                          "(default: 2000 for the membrane, 0 otherwise)")
@@ -506,7 +516,7 @@ def main():
         # replays below start again; the timed steps continue the run in the
         # same engine after RESUME_STEPS more steps
         t = time.perf_counter()
-        e = relax(w, Engine, relax_steps, skin=args.skin)
+        e = relax(w, Engine, relax_steps, skin=args.skin, state=args.state)
         w.text += (f"; the state after {relax_steps + RESUME_STEPS} steps from the lamellar "
                    "construction")
         ckpt = os.path.join(tempfile.gettempdir(), f"mdpd_bench_{os.getpid()}.ckpt")
