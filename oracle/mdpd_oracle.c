@@ -626,6 +626,53 @@ uint32_t oracle_head_conversion(const mdpd_params* p, uint32_t n,
 /* ======================================================================
  * Whole system.
  * ====================================================================== */
+/* ----------------------------------------------------------------------
+ * MT19937, the generator behind the reference's PS::MT (FDPS wraps the
+ * mt19937ar functions init_genrand / genrand_int32 / genrand_real1 /
+ * genrand_real3, chemmanager.hpp:19,74,154,167,178).  Restated from the
+ * published algorithm (M. Matsumoto and T. Nishimura, "Mersenne Twister: a
+ * 623-dimensionally equidistributed uniform pseudo-random number
+ * generator", ACM TOMACS 8 (1998) 3-30, with the 2002 initialisation):
+ * state of 624 words, twist with the matrix word 0x9908b0df, tempering by
+ * (11, 7 & 0x9d2c5680, 15 & 0xefc60000, 18).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  uint32_t mt[624];
+  int mti;
+} o_mt;
+
+static void o_mt_init(o_mt* g, uint32_t seed) {
+  g->mt[0] = seed;
+  for (int i = 1; i < 624; ++i)
+    g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+  g->mti = 624;
+}
+
+static uint32_t o_mt_u32(o_mt* g) {
+  if (g->mti >= 624) {
+    for (int k = 0; k < 624; ++k) {
+      const uint32_t y = (g->mt[k] & 0x80000000u) | (g->mt[(k + 1) % 624] & 0x7fffffffu);
+      g->mt[k] = g->mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    g->mti = 0;
+  }
+  uint32_t y = g->mt[g->mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+/* genrand_real1: [0, 1]; genrand_real3: (0, 1) */
+static double o_mt_real1(o_mt* g) { return (double)o_mt_u32(g) * (1.0 / 4294967295.0); }
+static double o_mt_real3(o_mt* g) { return ((double)o_mt_u32(g) + 0.5) * (1.0 / 4294967296.0); }
+
+void oracle_mt_u32(uint32_t seed, uint32_t n, uint32_t* out) {
+  o_mt g;
+  o_mt_init(&g, seed);
+  for (uint32_t k = 0; k < n; ++k) out[k] = o_mt_u32(&g);
+}
+
 struct oracle_sys {
   mdpd_params p;
   uint32_t n, cap, n_amp, amp_cap;
@@ -634,7 +681,19 @@ struct oracle_sys {
   double* cf_bend;
   uint32_t time;
   double bonded_vir[3];
+  /* reference-MT mode of the chemistry: one MT19937 stream consumed in the
+     reference's order (ChemManager(seed)); mt_draws counts its outputs */
+  int mt_on;
+  o_mt mt;
+  uint64_t mt_draws;
 };
+
+void oracle_sys_set_mt(oracle_sys* s, uint32_t seed, int32_t enable) {
+  s->mt_on = enable != 0;
+  o_mt_init(&s->mt, seed);
+  s->mt_draws = 0;
+}
+uint64_t oracle_sys_mt_draws(const oracle_sys* s) { return s->mt_draws; }
 
 oracle_sys* oracle_sys_new(const mdpd_params* p, uint32_t cap, uint32_t amp_cap) {
   oracle_sys* s = (oracle_sys*)calloc(1, sizeof(oracle_sys));
@@ -811,8 +870,13 @@ int oracle_sys_chem_event(oracle_sys* s, const mdpd_chem_params* cp, int32_t* st
       const double rad = sqrt(o_dot(tang, tang));
       const double rad_thr = cp->influ_grd * depth + cp->influ_rad;
       if ((rad <= rad_thr) && (depth >= -cp->influ_hei) && (depth <= cp->influ_dep)) {
-        o_saru sr = o_saru3(j, s->time, s->p.chem_seed + TAG_CONE * (c + 1));
-        rnd = o_saru_uni(&sr);
+        if (s->mt_on) {
+          rnd = o_mt_real1(&s->mt);  /* chemmanager.hpp:153-154 */
+          s->mt_draws++;
+        } else {
+          o_saru sr = o_saru3(j, s->time, s->p.chem_seed + TAG_CONE * (c + 1));
+          rnd = o_saru_uni(&sr);
+        }
       }
       if (rnd <= cp->p_thresld) {
         if (ntarget == tcap) { tcap *= 2; tgt = (uint32_t*)realloc(tgt, 2 * tcap * sizeof(uint32_t)); }
@@ -837,8 +901,14 @@ int oracle_sys_chem_event(oracle_sys* s, const mdpd_chem_params* cp, int32_t* st
     o_min_image_round(box, h2e);
     o_normalize(h2e);
     const uint32_t new_amp = amp0 + t;
-    o_saru st = o_saru3(new_amp, s->time, s->p.chem_seed ^ TAG_TANG);
-    const double theta = 2.0 * M_PI * o_saru_uni(&st);
+    double theta;
+    if (s->mt_on) {
+      theta = 2.0 * M_PI * o_mt_real1(&s->mt);  /* GetMembraneTangRandomVec, :167 */
+      s->mt_draws++;
+    } else {
+      o_saru st = o_saru3(new_amp, s->time, s->p.chem_seed ^ TAG_TANG);
+      theta = 2.0 * M_PI * o_saru_uni(&st);
+    }
     double base[3] = {cos(theta), sin(theta), 0.0};
     const double bd = o_dot(base, h2e);
     double tv[3];
@@ -859,7 +929,16 @@ int oracle_sys_chem_event(oracle_sys* s, const mdpd_chem_params* cp, int32_t* st
       o_saru sv = o_saru3(new_id, s->time, s->p.chem_seed ^ TAG_VEL);
       for (int a = 0; a < 3; ++a) {
         s->pos[3 * i + a] = np_[a];
-        s->vel[3 * i + a] = o_saru_nrml(&sv);  /* sqrt(-2 T log u1) cos(2 pi u2), T = 1 */
+        if (s->mt_on) {
+          /* GenThermalVeloc (:18-20), T = 1: sqrt(-2 log u) cos(2 pi u') with
+             u, u' two genrand_real3 draws; C++ leaves the order of the two
+             calls in that expression unspecified, here the log's comes first */
+          const double u = o_mt_real3(&s->mt), u2 = o_mt_real3(&s->mt);
+          s->mt_draws += 2;
+          s->vel[3 * i + a] = sqrt(-2.0 * log(u)) * cos(2.0 * M_PI * u2);
+        } else {
+          s->vel[3 * i + a] = o_saru_nrml(&sv);  /* sqrt(-2 T log u1) cos(2 pi u2), T = 1 */
+        }
         s->vel_buf[3 * i + a] = 0.0;
         s->acc[3 * i + a] = 0.0;
         s->press[3 * i + a] = 0.0;

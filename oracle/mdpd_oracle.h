@@ -22,9 +22,16 @@
  *                   and finite differences of the bond/bend energies
  *   drift/kick      exact algebra of driftkick.hpp:5-54
  *   observe         direct sums
- *   chemistry       invariants (ids, counts, geometry); the reference's MT
- *                   stream is replaced by keyed Saru draws: "parity unpinned"
- *                   against the reference's MT sequence (PS::MT is absent).
+ *   chemistry       constructed cones, brute-force candidate counts and
+ *                   binomial statistics (ids, counts, geometry) with the
+ *                   keyed Saru draws the engine uses; a reference-MT mode
+ *                   consumes one MT19937 stream in the reference's order
+ *                   (the generator pinned by its published 10000th output
+ *                   and numpy's legacy MT19937; the mode by its draw count
+ *                   and by matching the keyed mode wherever the draws do
+ *                   not decide).  The keyed draws themselves: "parity
+ *                   unpinned" against the reference's MT sequence, which
+ *                   the reference seeds per run.
  *   make_config     closed-form geometry of each mode (bond vectors, leaflet
  *                   heights and radii, ring counts floor(2 pi r sin / lip),
  *                   per-100 leaflet split), Box-Muller over the pinned Saru
@@ -123,6 +130,15 @@ void oracle_sys_remove_cm_drift(oracle_sys* s);
 uint32_t oracle_sys_head_conversion(oracle_sys* s);
 int oracle_sys_chem_event(oracle_sys* s, const mdpd_chem_params* cp,
                           int32_t* stop);
+/* reference-MT mode of oracle_sys_chem_event (enable != 0): the draws come
+   from one MT19937 stream seeded like ChemManager(seed) and consumed in the
+   reference's order (cone test per core and amphiphile, then per target the
+   tangent angle and the new beads' velocities); enable = 0 restores the
+   keyed draws.  oracle_sys_mt_draws counts the stream's outputs. */
+void oracle_sys_set_mt(oracle_sys* s, uint32_t seed, int32_t enable);
+uint64_t oracle_sys_mt_draws(const oracle_sys* s);
+/* the first n genrand_int32 outputs of MT19937 after init_genrand(seed) */
+void oracle_mt_u32(uint32_t seed, uint32_t n, uint32_t* out);
 void oracle_sys_observe(oracle_sys* s, uint32_t sol_num,
                         mdpd_observables* out);
 /* nsteps of the main loop body; returns 0 or 1 if chemistry asked to stop */

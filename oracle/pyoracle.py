@@ -58,6 +58,9 @@ _PROTOS = {
     "oracle_sys_remove_cm_drift": (None, [VP]),
     "oracle_sys_head_conversion": (C.c_uint32, [VP]),
     "oracle_sys_chem_event": (C.c_int, [VP, P(MdpdChemParams), P(C.c_int32)]),
+    "oracle_sys_set_mt": (None, [VP, C.c_uint32, C.c_int32]),
+    "oracle_sys_mt_draws": (C.c_uint64, [VP]),
+    "oracle_mt_u32": (None, [C.c_uint32, C.c_uint32, _U32P]),
     "oracle_sys_observe": (None, [VP, C.c_uint32, P(MdpdObservables)]),
     "oracle_sys_run": (C.c_int, [VP, C.c_uint32]),
     "oracle_num_threads": (C.c_int, []),
@@ -286,6 +289,14 @@ class System:
     def head_conversion(self):
         return lib().oracle_sys_head_conversion(self.h)
 
+    def set_mt(self, seed, enable=True):
+        """Reference-MT mode of chem_event: one MT19937 stream seeded like
+        ChemManager(seed), consumed in the reference's order."""
+        lib().oracle_sys_set_mt(self.h, seed, 1 if enable else 0)
+
+    def mt_draws(self):
+        return int(lib().oracle_sys_mt_draws(self.h))
+
     def chem_event(self, cp):
         stop = C.c_int32(0)
         k = lib().oracle_sys_chem_event(self.h, C.byref(cp), C.byref(stop))
@@ -322,3 +333,10 @@ def make_config(spec):
     kt = C.c_double()
     rc = lib().oracle_make_config(C.byref(spec), n, _d(pos), _d(vel), *[_u(a) for a in u], C.byref(kt))
     return rc, dict(pos=pos, vel=vel, ids=u[0], prop=u[1], amp_id=u[2], unit=u[3], kin_temp=kt.value)
+
+
+def mt_u32(seed, n):
+    """The first n genrand_int32 outputs of MT19937 after init_genrand(seed)."""
+    out = np.zeros(n, np.uint32)
+    lib().oracle_mt_u32(seed, n, _u(out))
+    return out

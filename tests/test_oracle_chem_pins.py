@@ -128,8 +128,8 @@ def test_constructed_cone_targets_and_geometry(oracle):
     s = _constructed()
     # the CM sums the target test starts from, by brute force
     hl, hb, _, _ = cm_brute(s["pos"], s["prop"], BOX)
-    assert np.allclose(hb[0], (H + [0, 0, -0.75] + SHIFT) % BOX, atol=1e-12)
-    assert np.allclose(hl[0], (H + [0.15, 0, 0.3] + SHIFT) % BOX, atol=1e-12)
+    assert np.allclose(hb[0], (H + [0, 0, -0.75] + SHIFT) % BOX, atol=1e-12, rtol=0)
+    assert np.allclose(hl[0], (H + [0.15, 0, 0.3] + SHIFT) % BOX, atol=1e-12, rtol=0)
     n0, amp0 = len(s["pos"]), len(s["topo"]) // 4
     eps = 0.2
     p, o, k, stop = _run_event(oracle, s, _chem([0], 1.0, eps=eps))
@@ -153,7 +153,7 @@ def test_constructed_cone_targets_and_geometry(oracle):
         assert math.isclose(np.linalg.norm(t), 1.0, rel_tol=1e-12)
         assert abs(t @ h_hat) < 1e-12
         for u in range(4):
-            assert np.allclose(d[u], eps * t + u * 0.5 * h_hat, atol=1e-12)
+            assert np.allclose(d[u], eps * t + u * 0.5 * h_hat, atol=1e-12, rtol=0)
         assert np.all((r["pos"][new] >= 0) & (r["pos"][new] < BOX))
         assert np.all(np.isfinite(r["vel"][new]))
         assert np.all(r["delta"][new] == 0) and np.all(r["vel_buf"][new] == 0)
@@ -182,7 +182,7 @@ def test_tangent_follows_keyed_draw(oracle):
         t = b - h * (b @ h)
         t /= np.linalg.norm(t)
         d = _minimg(r["pos"][n0 + 4 * i] - s["pos"][4 * a], BOX)
-        assert np.allclose(d, 0.3 * t, atol=1e-12)
+        assert np.allclose(d, 0.3 * t, atol=1e-12, rtol=0)
 
 
 def test_threshold_zero_and_two_cores(oracle):
@@ -295,3 +295,100 @@ def test_target_count_is_binomial(oracle):
         total += k
     mean, sd = trials * m * pthr, math.sqrt(trials * m * pthr * (1 - pthr))
     assert abs(total - mean) < 5 * sd, (total, mean, sd)
+
+
+# ------------------------------------------------------ reference-MT mode
+def test_mt19937_published_values(oracle):
+    """The oracle's MT19937 (the generator behind PS::MT): the 10000th
+    genrand_int32 output after init_genrand(5489) is 4123659995 (the check
+    value of the published generator), and the raw outputs equal numpy's
+    legacy MT19937 (RandomState seeds with init_genrand)."""
+    a = oracle.mt_u32(5489, 10000)
+    assert int(a[-1]) == 4123659995
+    for seed in (0, 1, 77, 12345):
+        ref = np.random.RandomState(seed).randint(0, 2 ** 32, size=2000, dtype=np.uint32)
+        assert np.array_equal(oracle.mt_u32(seed, 2000), ref)
+
+
+def oracle_mt(seed, n):
+    from oracle import pyoracle
+    return pyoracle.mt_u32(seed, n)
+
+
+def test_reference_mt_mode_order_and_draws(oracle):
+    """RandomChemEvent with one MT19937 stream, as ChemManager(seed) drives
+    it (chemmanager.hpp:116-173, 244-281): one genrand_real1 per in-cone
+    head, in core then amphiphile order, compared with p_thresld; per target
+    one genrand_real1 for the tangent angle and, per new bead and axis, two
+    genrand_real3 for GenThermalVeloc (:18-20).  The targets, their order
+    and the insertion geometry are those of the keyed mode wherever the
+    draws do not decide (p_thresld = 1 and 0); the tangent angles and the
+    velocities are recomputed here from the same stream."""
+    s = _constructed()
+    n0, amp0 = len(s["pos"]), len(s["topo"]) // 4
+    expect = [0] + [1 + i for i, (_, inside) in enumerate(HEADS) if inside]
+    seed = 2024
+    for pth in (1.0, 0.0):
+        _, ok_, kk, _ = _run_event(oracle, s, _chem([0], pth))
+        p = _params(BOX)
+        o = oracle.System(p, n0 + 4 * 64, amp0 + 64)
+        o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+        o.set_topology(s["topo"], core_amp_id=[0])
+        o.set_clock(40)
+        o.set_mt(seed)
+        k, stop = o.chem_event(_chem([0], pth, eps=0.2))
+        assert k == kk == (len(expect) if pth == 1.0 else 0)
+        assert o.mt_draws() == len(expect) + k * (1 + 4 * 3 * 2)
+        r, rk = o.get(), ok_.get()
+        for key in ("id", "prop", "amp_id", "unit"):
+            assert np.array_equal(r[key], rk[key]), key
+        if k == 0:
+            continue
+        u32 = oracle_mt(seed, o.mt_draws()).astype(np.float64)
+        q = len(expect)  # the cone draws come first
+        for i, a in enumerate(expect):
+            th = 2 * math.pi * (u32[q] * (1.0 / 4294967295.0))
+            q += 1
+            b = np.array([math.cos(th), math.sin(th), 0.0])
+            h = s["dirs"][a]
+            t = b - h * (b @ h)
+            t /= np.linalg.norm(t)
+            d = _minimg(r["pos"][n0 + 4 * i:n0 + 4 * i + 4] - s["pos"][4 * a], BOX)
+            for uu in range(4):
+                assert np.allclose(d[uu], 0.2 * t + uu * 0.5 * h, atol=1e-12, rtol=0)
+            for uu in range(4):
+                for ax in range(3):
+                    r3a = (u32[q] + 0.5) * (1.0 / 4294967296.0)
+                    r3b = (u32[q + 1] + 0.5) * (1.0 / 4294967296.0)
+                    q += 2
+                    v = math.sqrt(-2.0 * math.log(r3a)) * math.cos(2.0 * math.pi * r3b)
+                    assert math.isclose(r["vel"][n0 + 4 * i + uu, ax], v, rel_tol=1e-14, abs_tol=1e-15)
+        assert q == o.mt_draws()
+
+
+def test_reference_mt_mode_threshold_statistics(oracle):
+    """With p_thresld = 0.3 the MT mode takes each in-cone head with its own
+    genrand_real1 <= 0.3: the targets are exactly the in-cone heads whose
+    draw (in stream order) passes, recomputed here from the stream."""
+    s = _constructed()
+    expect = [0] + [1 + i for i, (_, inside) in enumerate(HEADS) if inside]
+    hits = 0
+    for seed in range(40):
+        p = _params(BOX)
+        n0, amp0 = len(s["pos"]), len(s["topo"]) // 4
+        o = oracle.System(p, n0 + 4 * 64, amp0 + 64)
+        o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+        o.set_topology(s["topo"], core_amp_id=[0])
+        o.set_clock(40)
+        o.set_mt(seed)
+        k, _ = o.chem_event(_chem([0], 0.3))
+        u = oracle_mt(seed, len(expect)).astype(np.float64) * (1.0 / 4294967295.0)
+        chosen = [a for a, x in zip(expect, u) if x <= 0.3]
+        assert k == len(chosen)
+        r = o.get()
+        for i, a in enumerate(chosen):
+            d = _minimg(r["pos"][n0 + 4 * i] - s["pos"][4 * a], BOX)
+            assert abs(np.linalg.norm(d) - 0.2) < 1e-12   # eps from the chosen head
+        hits += k
+    # 40 events x 5 candidates at p = 0.3: mean 60, sd 6.5
+    assert 30 < hits < 90
