@@ -26,6 +26,7 @@ the FP64 oracle (the reference's per-bead gather loops, OpenMP) timed on a
 bounded sample of the same workload with every host thread and with one.
 """
 import argparse
+import atexit
 import json
 import math
 import os
@@ -520,6 +521,7 @@ def main():
         w.text += (f"; the state after {relax_steps + RESUME_STEPS} steps from the lamellar "
                    "construction")
         ckpt = os.path.join(tempfile.gettempdir(), f"mdpd_bench_{os.getpid()}.ckpt")
+        atexit.register(lambda: os.path.exists(ckpt) and os.remove(ckpt))
         e.checkpoint_save(ckpt)
         e.run(RESUME_STEPS)
         log(f"[bench] relaxed the constructed state for {relax_steps} + {RESUME_STEPS} steps: "
@@ -726,8 +728,6 @@ def main():
     }
     print(json.dumps(out), flush=True)
     e.close()
-    if ckpt:
-        os.remove(ckpt)
 
 
 if __name__ == "__main__":

@@ -471,15 +471,18 @@ int capture_variant(mdpd_ctx* c, int p, int v) {
   const int p0 = c->parity;
   if (c->parity != p) swap_master(c);
   hipGraph_t g = nullptr;
-  HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  enqueue_step(c, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0, (v & 8) != 0);  // flips the parity
-  HIPCHK(c, hipStreamEndCapture(c->st, &g));
-  const hipError_t e = hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    enqueue_step(c, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0, (v & 8) != 0);  // flips the parity
+    e = hipStreamEndCapture(c->st, &g);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec[p][v], g, nullptr, nullptr, 0);
+  if (g) hipGraphDestroy(g);
+  // the host's view of the master sets is the one it had, on every path
   if (c->parity != p0) swap_master(c);
   if (e != hipSuccess) {
     c->gexec[p][v] = nullptr;
-    return fail(c, MDPD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return fail(c, MDPD_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
   }
   return MDPD_OK;
 }
