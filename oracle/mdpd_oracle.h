@@ -29,9 +29,10 @@
  *                   (the generator pinned by its published 10000th output
  *                   and numpy's legacy MT19937; the mode by its draw count
  *                   and by matching the keyed mode wherever the draws do
- *                   not decide).  The keyed draws themselves: "parity
- *                   unpinned" against the reference's MT sequence, which
- *                   the reference seeds per run.
+ *                   not decide); on one process the reference seeds the
+ *                   stream with time(nullptr) * rank = 0 (main.cpp:150),
+ *                   which seed 0 replays.  The keyed draws themselves:
+ *                   "parity unpinned" against the reference's MT sequence.
  *   make_config     closed-form geometry of each mode (bond vectors, leaflet
  *                   heights and radii, ring counts floor(2 pi r sin / lip),
  *                   per-100 leaflet split), Box-Muller over the pinned Saru
