@@ -647,15 +647,17 @@ def main():
                 "achieved_gbs": round(value * bstep / 1e9, 1),
                 "frac": round(value * bstep / (HBM_PEAK_GBS * 1e9), 4)}
 
-    chem = None
-    if w.bonded and not w.ensemble and args.skin == 0 and not args.no_chem:
-        # RandomChemEvent inside the captured steps from the same start: four
-        # core amphiphiles, cones of radius 3, depth -1..3 around them
+    def chem_leg(mt):
+        """RandomChemEvent inside the captured steps from the same start: four
+        core amphiphiles, cones of radius 3, depth -1..3 around them; keyed
+        draws, or (mt) the reference's MT19937 stream seeded with 0."""
         restart(e)
         core = np.array([4 * a for a in (0, 1, 2, 3)], np.uint32)
         n_amp0 = e.observe()["n_amp"]
         e.set_chemistry(p_thresld=2e-3, eps=0.2, influ_rad=3.0, influ_hei=1.0, influ_dep=3.0,
                         influ_grd=0.0, max_amp_num=10 ** 9, core_ptcl_id=core, beg_chem=0)
+        if mt:
+            e.set_chem_mt(0)
         e.run(3)
         e.prepare_run(args.steps)  # the chemistry steps' graphs, before the clock starts
         e.sync()
@@ -663,6 +665,7 @@ def main():
         e.run(args.steps)
         cwall = time.perf_counter() - t
         stopped, events, inserted = e.chem_status()
+        draws = e.chem_mt_draws() if mt else None
         # a few more chemistry steps launched kernel by kernel: the event's
         # own time (the "reactions" phase)
         e.set_profiling(True)
@@ -670,18 +673,30 @@ def main():
         e.sync()
         cph = e.phase_times()
         e.set_profiling(False)
+        if mt:
+            e.set_chem_mt(enable=False)
         e.set_chemistry(0, 0, 0, 0, 0, 0, 0, [], enable=False)
-        chem = {"ms_per_step": round(cwall * 1e3 / args.steps, 5),
-                "ms_per_step_without": round(ms_step, 5),
-                "event_ms_eager": round(cph.get("reactions", float("nan")), 5),
-                "events": events, "amphiphiles_inserted": inserted,
-                "beads_end": e.num_beads(), "n_amp_start": n_amp0,
-                "setup": "RandomChemEvent (cone target test, keyed draws) at the end of every "
-                         "step inside the captured steps: cores = amphiphiles 0-3, p_thresld 2e-3, "
-                         "influ_rad 3, influ_hei 1, influ_dep 3, influ_grd 0, eps 0.2; mdpd_run "
-                         "synchronises once at its end to read back the bead count"}
-        log(f"[bench] chemistry on: {chem['ms_per_step']:.4f} ms/step, {inserted} amphiphiles "
-            f"inserted in {events} events")
+        res = {"ms_per_step": round(cwall * 1e3 / args.steps, 5),
+               "ms_per_step_without": round(ms_step, 5),
+               "event_ms_eager": round(cph.get("reactions", float("nan")), 5),
+               "events": events, "amphiphiles_inserted": inserted,
+               "beads_end": e.num_beads(), "n_amp_start": n_amp0,
+               "setup": "RandomChemEvent (cone target test, %s) at the end of every "
+                        "step inside the captured steps: cores = amphiphiles 0-3, p_thresld 2e-3, "
+                        "influ_rad 3, influ_hei 1, influ_dep 3, influ_grd 0, eps 0.2; mdpd_run "
+                        "synchronises once at its end to read back the bead count"
+                        % ("the reference's MT19937 stream seeded with 0, in its draw order"
+                           if mt else "keyed draws")}
+        if mt:
+            res["mt_draws"] = draws
+        log(f"[bench] chemistry on{' (reference MT)' if mt else ''}: {res['ms_per_step']:.4f} "
+            f"ms/step, {inserted} amphiphiles inserted in {events} events")
+        return res
+
+    chem = chem_mt = None
+    if w.bonded and not w.ensemble and args.skin == 0 and not args.no_chem:
+        chem = chem_leg(False)
+        chem_mt = chem_leg(True)
 
     cpu = None
     if not args.no_cpu and not w.ensemble:
@@ -723,6 +738,7 @@ def main():
         "roofline": roof,
         "step_hbm_roofline": step_hbm,
         "chemistry_on": chem,
+        "chemistry_reference_mt": chem_mt,
         "cpu_baseline": cpu,
         "calibration": calibration_record(),
     }

@@ -196,6 +196,8 @@ PROTOTYPES = {
     "mdpd_density_profile": (C.c_int, [VP, C.c_int32, C.c_int32, _DP]),
     "mdpd_set_chemistry": (C.c_int, [VP, C.POINTER(MdpdChemParams), C.c_uint32, C.c_int32]),
     "mdpd_chem_status": (C.c_int, [VP, C.POINTER(C.c_int32), _U32P, _U32P]),
+    "mdpd_set_chem_mt": (C.c_int, [VP, C.c_uint32, C.c_int32]),
+    "mdpd_chem_mt_draws": (C.c_int, [VP, P(C.c_uint64)]),
     "mdpd_rdf": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
     "mdpd_rdf_noalloc": (C.c_int, [VP, C.c_double, C.c_int32, _DP]),
     "mdpd_write_xyz_async": (C.c_int, [VP, C.c_char_p, C.c_uint32, C.c_int32]),

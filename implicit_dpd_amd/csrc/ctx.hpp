@@ -55,6 +55,11 @@ struct mdpd_ctx {
   uint32_t* chem_targets = nullptr;       // [amp_cap]
   uint32_t* run_ok_word = nullptr;        // the device's run word of grow mode
   mdpd::ChemDev chem_host[2] = {};        // host copies (sources of the uploads)
+  // reference-MT mode (mdpd_set_chem_mt): chem_mt is the buffers of
+  // mt_store when on, all null (keyed draws) when off; mt_store is allocated
+  // on the first switch-on and kept
+  mdpd::ChemMt chem_mt = {};
+  mdpd::ChemMt mt_store = {};
   bool grow = false;          // chemistry may append beads: see DevState::nl
   bool chem_on = false;       // mdpd_run ends steps with time >= beg_chem with an event
   bool chem_stopped = false;  // an in-run event reached max_amp_num
@@ -90,7 +95,7 @@ int pull_counts(mdpd_ctx* c);
 uint32_t chem_count_blocks(uint32_t cap);
 void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
                        uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
-                       hipStream_t st);
+                       const ChemMt& mt, hipStream_t st);
 }  // namespace mdpd
 
 #define HIPCHK(c, call)                                                                    \

@@ -434,7 +434,7 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first, bool chem) {
   // cell keys with the others' when this step's force pass binned them)
   if (chem)
     launch_chem_event(d, c->chem_dev, c->chem_state, c->chem_host[0].n_core, c->chem_counts,
-                      c->chem_targets, c->d_amp_num, !virial && !cm, st);
+                      c->chem_targets, c->d_amp_num, !virial && !cm, c->chem_mt, st);
   mark(9);
   mark(10);
 }
@@ -1202,6 +1202,52 @@ int mdpd_set_chemistry(mdpd_ctx* c, const mdpd_chem_params* cp, uint32_t beg_che
   c->chem_on = true;
   c->chem_stopped = false;
   c->beg_chem = beg_chem;
+  return MDPD_OK;
+}
+
+int mdpd_set_chem_mt(mdpd_ctx* c, uint32_t seed, int32_t enable) {
+  if (!c) return MDPD_ERR_ARG;
+  if (c->n_rep != 1 || !c->chem_dev)
+    return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
+  if (!enable) {
+    if (c->chem_mt.g) invalidate_graphs(c);  // the captured events draw from the stream
+    c->chem_mt = ChemMt{};
+    return MDPD_OK;
+  }
+  int rc;
+  ChemMt& m = c->mt_store;
+  if (!m.g) {
+    const uint32_t amp_cap = c->cap / 4u + 1u;
+    m.cone_words = (uint32_t)(((uint64_t)kMaxCore * amp_cap) / 32u + 2u);
+    if ((rc = dalloc(c, &m.g, 1)) || (rc = dalloc(c, &m.cone_offs, (size_t)kMaxCore * chem_count_blocks(c->cap))) ||
+        (rc = dalloc(c, &m.cone_bits, m.cone_words)) ||
+        (rc = dalloc(c, &m.tdraws, (size_t)kMtDrawsPerTarget * amp_cap))) {
+      m = ChemMt{};
+      return rc;
+    }
+  }
+  // init_genrand (mt19937ar): the state ChemManager(seed) starts from
+  MtState h;
+  h.mt[0] = seed;
+  for (uint32_t i = 1; i < kMtN; ++i) h.mt[i] = 1812433253u * (h.mt[i - 1] ^ (h.mt[i - 1] >> 30)) + i;
+  h.mti = kMtN;
+  h.pad = 0;
+  h.draws = 0;
+  HIPCHK(c, hipMemcpyAsync(m.g, &h, sizeof(MtState), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (!c->chem_mt.g) invalidate_graphs(c);
+  c->chem_mt = m;
+  return MDPD_OK;
+}
+
+int mdpd_chem_mt_draws(mdpd_ctx* c, uint64_t* draws) {
+  if (!c || !draws) return MDPD_ERR_ARG;
+  *draws = 0;
+  if (!c->chem_mt.g) return MDPD_OK;
+  MtState h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->chem_mt.g, sizeof(MtState), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *draws = h.draws;
   return MDPD_OK;
 }
 

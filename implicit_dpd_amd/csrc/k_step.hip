@@ -983,7 +983,13 @@ void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3
 // keyed Saru streams: the cone test (amphiphile j, time, chem_seed +
 // TAG_CONE (core + 1)), the tangent angle (new amphiphile id, time,
 // chem_seed ^ TAG_TANG), the velocities (new bead id, time, chem_seed ^
-// TAG_VEL).  `time` is the device clock.
+// TAG_VEL).  `time` is the device clock.  In the reference-MT mode (ChemMt)
+// the draws are the outputs of one MT19937 stream instead, in the
+// reference's order: k_chem_count first counts the heads inside each cone,
+// k_chem_scan turns the counts into each head's rank, k_chem_mt draws one
+// genrand_real1 per in-cone head in rank order, k_chem_count / k_chem_scatter
+// read the decisions by rank, and after the scan k_chem_mt draws the
+// targets' 25 outputs each for k_chem_insert.
 constexpr int kChemBlock = 256;
 
 __global__ void __launch_bounds__(256) k_chem_find(DevState d, const ChemDev* __restrict__ cd,
@@ -1086,10 +1092,11 @@ __global__ void __launch_bounds__(kCoreBlock) k_chem_core(DevState d, const Chem
   }
 }
 
-// DetermineTargetIdWithCorePos loop body (chemmanager.hpp:140-161)
-__device__ __forceinline__ bool chem_target(const DevState& d, const ChemDev* __restrict__ cd,
-                                            const ChemState* __restrict__ cs, uint32_t c,
-                                            uint32_t j, uint32_t time) {
+// DetermineTargetIdWithCorePos loop body (chemmanager.hpp:140-161): is the
+// head of chain j inside the truncated cone of core c?
+__device__ __forceinline__ bool chem_in_cone(const DevState& d, const ChemDev* __restrict__ cd,
+                                             const ChemState* __restrict__ cs, uint32_t c,
+                                             uint32_t j) {
 #pragma clang fp contract(off)
   const Replica& R = d.reps[0];
   const double* ch = cs->core_geo[c];
@@ -1102,28 +1109,66 @@ __device__ __forceinline__ bool chem_target(const DevState& d, const ChemDev* __
   for (int a = 0; a < 3; ++a) tg[a] = c2p[a] - depth * h2t[a];
   const double rad = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
   const double rad_thr = cd->influ_grd * depth + cd->influ_rad;
-  double rnd = 2.0;
-  if ((rad <= rad_thr) && (depth >= -cd->influ_hei) && (depth <= cd->influ_dep)) {
-    Saru s = saru_seed(j, time, R.chem_seed + TAG_CONE * (c + 1u));
-    rnd = saru_uni_f64(s);
-  }
-  return rnd <= cd->p_thresld;
+  return (rad <= rad_thr) && (depth >= -cd->influ_hei) && (depth <= cd->influ_dep);
 }
 
+// rank of this thread's flag among the set flags of the block, threads in
+// order (every thread of the block calls it; wcnt: kChemBlock / 64 words of
+// LDS not in use by the block)
+__device__ __forceinline__ uint32_t chem_block_rank(bool f, uint32_t* wcnt) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(f);
+  if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t r = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+  for (int k = 0; k < w; ++k) r += wcnt[k];
+  return r;
+}
+
+// the flag of chain j for core c, thread j % kChemBlock of block (j /
+// kChemBlock, c): kMode 0 keyed target, 1 head inside the cone, 2
+// reference-MT target (the in-cone draws decided in mt.cone_bits by rank).
+// A head outside the cone keeps rnd = 2.0, a target only if p_thresld >= 2.
+// Every thread of the block calls it.
+template <int kMode>
+__device__ __forceinline__ bool chem_flag(const DevState& d, const ChemDev* __restrict__ cd,
+                                          const ChemState* __restrict__ cs, const ChemMt& mt,
+                                          uint32_t c, uint32_t j, uint32_t* wcnt) {
+  const bool live = j < dev_namp(d);
+  const bool in = live && chem_in_cone(d, cd, cs, c, j);
+  if (kMode == 1) return in;
+  double rnd = 2.0;
+  if (kMode == 0) {
+    if (in) {
+      Saru s = saru_seed(j, *d.d_time, d.reps[0].chem_seed + TAG_CONE * (c + 1u));
+      rnd = saru_uni_f64(s);
+    }
+    return live && rnd <= cd->p_thresld;
+  }
+  const uint32_t r = mt.cone_offs[(size_t)c * gridDim.x + blockIdx.x] + chem_block_rank(in, wcnt);
+  if (in) return (mt.cone_bits[r >> 5] >> (r & 31u)) & 1u;
+  return live && rnd <= cd->p_thresld;
+}
+
+template <int kMode>
 __global__ void __launch_bounds__(kChemBlock) k_chem_count(DevState d, const ChemDev* __restrict__ cd,
                                                            const ChemState* __restrict__ cs,
-                                                           uint32_t* __restrict__ counts) {
+                                                           ChemMt mt, uint32_t* __restrict__ counts) {
   if (halted(d)) return;
+  __shared__ uint32_t wcnt[kChemBlock / 64];
   const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
-  const bool f = j < dev_namp(d) && chem_target(d, cd, cs, c, j, *d.d_time);
+  const bool f = chem_flag<kMode>(d, cd, cs, mt, c, j, wcnt);
   const int cnt = __syncthreads_count(f ? 1 : 0);
   if (threadIdx.x == 0) counts[(size_t)c * gridDim.x + blockIdx.x] = (uint32_t)cnt;
 }
 
 // one block of 1024 threads: counts[0..m) -> exclusive offsets, in order
+// cone: the counts are in-cone heads (reference-MT mode), whose total is
+// the number of draws of the cone tests
 __global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* __restrict__ cd,
                                                     ChemState* __restrict__ cs,
-                                                    uint32_t* __restrict__ counts, uint32_t m) {
+                                                    uint32_t* __restrict__ counts, uint32_t m,
+                                                    int cone) {
   if (halted(d)) return;
   __shared__ uint32_t wsum[16];
   const uint32_t per = (m + 1023u) / 1024u;
@@ -1145,7 +1190,8 @@ __global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* _
     counts[k] = run;
     run += v;
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && cone) cs->n_in = tot;
+  if (threadIdx.x == 0 && !cone) {
     const uint32_t n0 = dev_n(d), na0 = dev_namp(d);
     // amp_cap = cap / 4 + 1 chains fit the chain arrays
     const bool fits = (uint64_t)n0 + 4ull * tot <= (uint64_t)d.cap &&
@@ -1158,22 +1204,107 @@ __global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* _
   }
 }
 
+template <int kMode>
 __global__ void __launch_bounds__(kChemBlock) k_chem_scatter(DevState d, const ChemDev* __restrict__ cd,
                                                              const ChemState* __restrict__ cs,
+                                                             ChemMt mt,
                                                              const uint32_t* __restrict__ offs,
                                                              uint32_t* __restrict__ targets) {
   if (halted(d) || cs->nt == 0u) return;
-  __shared__ uint32_t wcnt[kChemBlock / 64];
+  __shared__ uint32_t wcnt[2][kChemBlock / 64];
   const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
-  const bool f = j < dev_namp(d) && chem_target(d, cd, cs, c, j, *d.d_time);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const unsigned long long bal = __ballot(f);
-  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-  if (lane == 0) wcnt[w] = (uint32_t)__popcll(bal);
+  const bool f = chem_flag<kMode>(d, cd, cs, mt, c, j, wcnt[0]);
+  const uint32_t r = chem_block_rank(f, wcnt[1]);
+  if (f) targets[offs[(size_t)c * gridDim.x + blockIdx.x] + r] = j;
+}
+
+// ---- MT19937 (mt19937ar: init_genrand / genrand_int32 of M. Matsumoto and
+// T. Nishimura, ACM TOMACS 8 (1998) 3-30, the 2002 initialisation), the
+// generator behind the reference's PS::MT.  One block keeps the 624-word
+// state in LDS and serves a stream of outputs in order.
+constexpr int kMtBlock = 640;  // ten waves: one thread per state word
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+// the twist of genrand_int32 (word k from words k, k + 1 and k + 397 mod 624
+// as they stand when the serial loop reaches k) in three bands of parallel
+// words: [0, 227) reads only old words; [227, 454) reads k - 227 from the
+// first band; [454, 624) reads k - 227 from the second and, for k = 623,
+// word 0 from the first.  Every thread of the block calls it.
+__device__ void mt_twist(uint32_t* s) {
+  const int t = threadIdx.x;
+  for (int band = 0; band < 3; ++band) {
+    const int k0 = band * 227, k1 = band == 2 ? (int)kMtN : k0 + 227;
+    const int k = k0 + t;
+    uint32_t v = 0;
+    if (k < k1) {
+      const uint32_t y = (s[k] & 0x80000000u) | (s[(k + 1) % (int)kMtN] & 0x7fffffffu);
+      v = s[(k + 397) % (int)kMtN] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    __syncthreads();
+    if (k < k1) s[k] = v;
+    __syncthreads();
+  }
+}
+// genrand_real1: [0, 1]; genrand_real3: (0, 1)
+__device__ __forceinline__ double mt_real1(uint32_t y) { return (double)y * (1.0 / 4294967295.0); }
+__device__ __forceinline__ double mt_real3(uint32_t y) {
+  return ((double)y + 0.5) * (1.0 / 4294967296.0);
+}
+
+// one block: the next outputs of the stream.  targets = 0: the n_in cone
+// draws, bit r of mt.cone_bits = (genrand_real1 of draw r <= p_thresld)
+// (the bits start cleared); targets = 1: the targets' 25 nt raw outputs
+// into mt.tdraws (none when the event was refused)
+__global__ void __launch_bounds__(kMtBlock) k_chem_mt(DevState d, const ChemDev* __restrict__ cd,
+                                                     const ChemState* __restrict__ cs, ChemMt mt,
+                                                     int targets) {
+#pragma clang fp contract(off)
+  if (halted(d)) return;
+  const uint32_t n = targets ? (cs->refused ? 0u : kMtDrawsPerTarget * cs->nt) : cs->n_in;
+  if (n == 0u) return;
+  __shared__ uint32_t s[kMtN];
+  MtState* g = mt.g;
+  const int t = threadIdx.x;
+  if (t < (int)kMtN) s[t] = g->mt[t];
+  uint32_t mti = g->mti;
   __syncthreads();
-  uint32_t base = offs[(size_t)c * gridDim.x + blockIdx.x];
-  for (int k = 0; k < w; ++k) base += wcnt[k];
-  if (f) targets[base + below] = j;
+  const double p = cd->p_thresld;
+  for (uint32_t done = 0; done < n;) {
+    if (mti >= kMtN) {
+      mt_twist(s);
+      mti = 0;
+    }
+    const uint32_t cnt = min(kMtN - mti, n - done);
+    const bool act = (uint32_t)t < cnt;
+    const uint32_t y = act ? mt_temper(s[mti + t]) : 0u;
+    if (targets) {
+      if (act) mt.tdraws[done + t] = y;
+    } else {
+      // 64 draws per wave: draws r0 .. r0 + 63 straddle at most three words
+      const unsigned long long bal = __ballot(act && mt_real1(y) <= p);
+      if ((t & 63) == 0 && bal) {
+        const uint32_t r0 = done + (uint32_t)t, w0 = r0 >> 5, sh = r0 & 31u;
+        const unsigned long long lo = bal << sh;
+        const uint32_t hi = sh ? (uint32_t)(bal >> (64u - sh)) : 0u;
+        if ((uint32_t)lo) atomicOr(&mt.cone_bits[w0], (uint32_t)lo);
+        if ((uint32_t)(lo >> 32)) atomicOr(&mt.cone_bits[w0 + 1], (uint32_t)(lo >> 32));
+        if (hi) atomicOr(&mt.cone_bits[w0 + 2], hi);
+      }
+    }
+    done += cnt;
+    mti += cnt;
+  }
+  __syncthreads();
+  if (t < (int)kMtN) g->mt[t] = s[t];
+  if (t == 0) {
+    g->mti = mti;
+    g->draws += n;
+  }
 }
 
 // one thread per new bead (grid-stride): target t makes amphiphile namp0 + t
@@ -1184,6 +1315,7 @@ __global__ void __launch_bounds__(kChemBlock) k_chem_scatter(DevState d, const C
 __global__ void __launch_bounds__(256) k_chem_insert(DevState d, const ChemDev* __restrict__ cd,
                                                      const ChemState* __restrict__ cs,
                                                      const uint32_t* __restrict__ targets,
+                                                     const uint32_t* __restrict__ tdraws,
                                                      int keys) {
 #pragma clang fp contract(off)
   if (halted(d)) return;
@@ -1206,8 +1338,16 @@ __global__ void __launch_bounds__(256) k_chem_insert(DevState d, const ChemDev* 
     double l = sqrt(h2e[0] * h2e[0] + h2e[1] * h2e[1] + h2e[2] * h2e[2]);
     for (int a = 0; a < 3; ++a) h2e[a] /= l;
     const uint32_t new_amp = na0 + t;
-    Saru st = saru_seed(new_amp, time, R.chem_seed ^ TAG_TANG);
-    const double theta = 2.0 * M_PI * saru_uni_f64(st);
+    // reference-MT mode: target t's outputs are tdraws[25 t ...]: the angle,
+    // then per bead u and axis a the log's and the cosine's draws
+    const uint32_t* td = tdraws ? tdraws + (size_t)kMtDrawsPerTarget * t : nullptr;
+    double theta;
+    if (td) {
+      theta = 2.0 * M_PI * mt_real1(td[0]);
+    } else {
+      Saru st = saru_seed(new_amp, time, R.chem_seed ^ TAG_TANG);
+      theta = 2.0 * M_PI * saru_uni_f64(st);
+    }
     double base[3] = {cos(theta), sin(theta), 0.0};
     const double bd = base[0] * h2e[0] + base[1] * h2e[1] + base[2] * h2e[2];
     double tv[3];
@@ -1227,8 +1367,14 @@ __global__ void __launch_bounds__(256) k_chem_insert(DevState d, const ChemDev* 
     d.chainpos[4 * (size_t)new_amp + u] = make_double4(np[0], np[1], np[2], 0.0);
     Saru sv = saru_seed(i, time, R.chem_seed ^ TAG_VEL);
     for (int a = 0; a < 3; ++a) {
-      const double u1 = saru_uni_f64(sv);
-      const double u2 = saru_uni_f64(sv);
+      double u1, u2;
+      if (td) {
+        u1 = mt_real3(td[1 + 6 * u + 2 * a]);
+        u2 = mt_real3(td[2 + 6 * u + 2 * a]);
+      } else {
+        u1 = saru_uni_f64(sv);
+        u2 = saru_uni_f64(sv);
+      }
       vv[a] = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
       d.vel[a * cap + i] = vv[a];
       d.vbuf[a * cap + i] = 0.0;
@@ -1279,23 +1425,46 @@ __global__ void k_chem_commit(DevState d, const ChemDev* __restrict__ cd,
 
 void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
                        uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
-                       hipStream_t st) {
-  const uint32_t nblk = div_up(d.cap / 4u + 1u, (uint32_t)kChemBlock);
+                       const ChemMt& mt, hipStream_t st) {
+  const uint32_t amp_cap = d.cap / 4u + 1u;
+  const uint32_t nblk = div_up(amp_cap, (uint32_t)kChemBlock);
+  const bool mtm = mt.g != nullptr;
+  const ChemState* ccs = cs;
   if (n_core) {
     hipLaunchKernelGGL(k_chem_find, grid1(d.nl, 256), dim3(256), 0, st, d, cd, cs);
     hipLaunchKernelGGL(k_chem_core, dim3(n_core), dim3(kCoreBlock), 0, st, d, cd, cs);
-    hipLaunchKernelGGL(k_chem_count, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
-                       (const ChemState*)cs, counts);
+    if (mtm) {
+      // at most one draw per (core, chain): n_core amp_cap bits
+      launch_fill_u32(mt.cone_bits, (uint32_t)(((uint64_t)n_core * amp_cap + 31u) / 32u) + 2u, 0u,
+                      st);
+      hipLaunchKernelGGL(k_chem_count<1>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
+                         mt, mt.cone_offs);
+      hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, mt.cone_offs,
+                         n_core * nblk, 1);
+      hipLaunchKernelGGL(k_chem_mt, dim3(1), dim3(kMtBlock), 0, st, d, cd, ccs, mt, 0);
+      hipLaunchKernelGGL(k_chem_count<2>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
+                         mt, counts);
+    } else {
+      hipLaunchKernelGGL(k_chem_count<0>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
+                         mt, counts);
+    }
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, counts, 1u, 0u);
   }
   const uint32_t m = n_core ? n_core * nblk : 1u;
-  hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, counts, m);
-  if (n_core)
-    hipLaunchKernelGGL(k_chem_scatter, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
-                       (const ChemState*)cs, (const uint32_t*)counts, targets);
-  hipLaunchKernelGGL(k_chem_insert, dim3(256), dim3(256), 0, st, d, cd, (const ChemState*)cs,
-                     (const uint32_t*)targets, keys ? 1 : 0);
+  hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, counts, m, 0);
+  if (n_core) {
+    if (mtm)
+      hipLaunchKernelGGL(k_chem_scatter<2>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
+                         ccs, mt, (const uint32_t*)counts, targets);
+    else
+      hipLaunchKernelGGL(k_chem_scatter<0>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
+                         ccs, mt, (const uint32_t*)counts, targets);
+  }
+  if (mtm) hipLaunchKernelGGL(k_chem_mt, dim3(1), dim3(kMtBlock), 0, st, d, cd, ccs, mt, 1);
+  hipLaunchKernelGGL(k_chem_insert, dim3(256), dim3(256), 0, st, d, cd, ccs,
+                     (const uint32_t*)targets, (const uint32_t*)(mtm ? mt.tdraws : nullptr),
+                     keys ? 1 : 0);
   hipLaunchKernelGGL(k_chem_commit, dim3(1), dim3(256), 0, st, d, cd, cs, amp_num);
 }
 uint32_t chem_count_blocks(uint32_t cap) { return div_up(cap / 4u + 1u, (uint32_t)kChemBlock); }

@@ -125,9 +125,32 @@ struct ChemState {
   uint32_t refused;  // it would have exceeded the capacity (nothing inserted)
   uint32_t events;   // events so far
   uint32_t total;    // amphiphiles inserted so far
-  uint32_t pad;
+  uint32_t n_in;     // reference-MT mode: heads inside a cone at the last event
   uint32_t core_slot[kMaxCore];   // storage slot of each core head (~0: absent)
   double core_geo[kMaxCore][6];   // corrected head position, membrane normal h2t
+};
+
+// The reference-MT mode of the chemistry: the draws come from one MT19937
+// stream (the reference's PS::MT, seeded by ChemManager(seed),
+// chemmanager.hpp:176-179, main.cpp:151) consumed in the reference's order: one
+// genrand_real1 per head inside a core's cone, cores in order and chains in
+// order within a core (:140-161), then per target one genrand_real1 for the
+// tangent angle (:167) and two genrand_real3 per velocity component of its
+// four beads (:18-20, the log's draw first).
+constexpr uint32_t kMtN = 624;
+constexpr uint32_t kMtDrawsPerTarget = 25;
+struct MtState {
+  uint32_t mt[kMtN];
+  uint32_t mti;                // index of the next output in mt (kMtN: twist first)
+  uint32_t pad;
+  unsigned long long draws;    // outputs consumed so far
+};
+struct ChemMt {
+  MtState* g;            // null: keyed draws
+  uint32_t* cone_offs;   // [kMaxCore * chem_count_blocks] in-cone heads, then their offsets
+  uint32_t* cone_bits;   // [cone_words] bit r: in-cone draw r is <= p_thresld
+  uint32_t* tdraws;      // [kMtDrawsPerTarget * amp_cap] the targets' raw outputs
+  uint32_t cone_words;   // kMaxCore * amp_cap / 32 + 2 (at most one draw per core and chain)
 };
 
 // launch wrappers (defined in the .hip files)

@@ -2,7 +2,8 @@
 // reference's src/main.cpp:77-209 (single process, no CALC_HEIGHT), written
 // against the C++ facade so that the calls read like the reference's:
 //
-//   mdpd_main <target_dir> [--seed S] [--chem] [--beg-chem T] [--nlist-width W]
+//   mdpd_main <target_dir> [--seed S] [--chem] [--chem-mt] [--beg-chem T]
+//             [--nlist-width W]
 //
 // Reads <target_dir>/run_param.txt and init_config.xyz, runs all_time steps
 // and writes kin_temp.txt, pressure.txt, diffus.txt, num_amp.txt,
@@ -10,7 +11,10 @@
 // CalcForceEpEpDPD::m_seed (the reference takes time(nullptr), main.cpp:133);
 // --chem enables the RandomChemEvent branch (the reference's CHEM_MODE) for
 // steps with time >= T (--beg-chem, default 100000 = Parameter::beg_chem,
-// parameter.hpp:254).  --nlist-width sets the engine's per-bead pair-list
+// parameter.hpp:254).  --chem-mt takes the chemistry's draws from the
+// reference's MT19937 stream in the reference's order, seeded as
+// ChemManager(seed) is on one process (time(nullptr) * rank = 0,
+// main.cpp:151), instead of the keyed draws.  --nlist-width sets the engine's per-bead pair-list
 // capacity (default 96) for dense systems.
 //
 // The main loop runs as the engine's production path: the steps between two
@@ -88,10 +92,11 @@ int main(int argc, char* argv[]) {
     std::abort();
   }
   const std::string cdir = argv[1];
-  bool chem = false, have_seed = false;
+  bool chem = false, chem_mt = false, have_seed = false;
   uint32_t seed = 0, beg_chem = kBegChem, nlist_width = 0;
   for (int i = 2; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--chem")) chem = true;
+    else if (!std::strcmp(argv[i], "--chem-mt")) chem_mt = true;
     else if (!std::strcmp(argv[i], "--beg-chem") && i + 1 < argc)
       beg_chem = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
     else if (!std::strcmp(argv[i], "--nlist-width") && i + 1 < argc)
@@ -178,6 +183,8 @@ int main(int argc, char* argv[]) {
   if (chem)
     mdpd::Check(mdpd_set_chemistry(system.ctx(), &cp, beg_chem, 1), system.ctx(),
                 "mdpd_set_chemistry");
+  if (chem && chem_mt)
+    mdpd::Check(mdpd_set_chem_mt(system.ctx(), 0u, 1), system.ctx(), "mdpd_set_chem_mt");
   const uint32_t atime = mdpd::Clock::time + in.all_time - 1;
   mdpd::Clock::Push(system);
   const uint32_t t_begin = mdpd::Clock::time;

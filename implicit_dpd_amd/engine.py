@@ -206,6 +206,19 @@ class Engine:
                             max_amp_num, len(ids), _u(ids) if len(ids) else None)
         self._c(self.L.mdpd_set_chemistry(self.h, C.byref(cp), int(beg_chem), 1 if enable else 0))
 
+    def set_chem_mt(self, seed=0, enable=True):
+        """Reference-MT draws for the chemistry: one MT19937 stream from
+        init_genrand(seed), consumed in the reference's order (ChemManager's
+        PS::MT; the reference's single-process run seeds it with 0).
+        enable=False returns to the keyed draws."""
+        self._c(self.L.mdpd_set_chem_mt(self.h, int(seed) & 0xffffffff, 1 if enable else 0))
+
+    def chem_mt_draws(self):
+        """Outputs of the reference-MT stream consumed since set_chem_mt."""
+        v = C.c_uint64()
+        self._c(self.L.mdpd_chem_mt_draws(self.h, C.byref(v)))
+        return v.value
+
     def chem_status(self):
         """(stopped, in-run events, amphiphiles inserted in runs)"""
         st, ev, ins = C.c_int32(), C.c_uint32(), C.c_uint32()

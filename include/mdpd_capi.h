@@ -203,7 +203,8 @@ int mdpd_bonded(mdpd_ctx* ctx, double vir[3]);  /* ForceBonded::CalcListedForce 
 int mdpd_kick(mdpd_ctx* ctx);                   /* driftkick.hpp:32-40 */
 int mdpd_remove_cm_drift(mdpd_ctx* ctx);        /* driftkick.hpp:42-54 */
 int mdpd_head_conversion(mdpd_ctx* ctx);        /* keyed type flip, no counterpart */
-/* RandomChemEvent (chemmanager.hpp:244-281) with keyed draws and the new
+/* RandomChemEvent (chemmanager.hpp:244-281) with keyed draws (or the
+   reference's MT19937 stream, mdpd_set_chem_mt) and the new
    amphiphiles appended to the bonded topology; *stop = 1 once
    amp_num >= max_amp_num.  The draws are keyed on the context clock.  The
    event runs on the device with the context's own scratch (no allocation,
@@ -224,6 +225,21 @@ int mdpd_chem_event(mdpd_ctx* ctx, const mdpd_chem_params* cp, int* stop);
    verlet_skin = 0; at most 256 core amphiphiles.  enable = 0 switches it off. */
 int mdpd_set_chemistry(mdpd_ctx* ctx, const mdpd_chem_params* cp, uint32_t beg_chem,
                        int32_t enable);
+/* Draws of the chemistry (mdpd_chem_event and the events inside mdpd_run):
+   enable != 0 takes them from one MT19937 stream started by
+   init_genrand(seed), consumed in the reference's order, as ChemManager's
+   PS::MT is (chemmanager.hpp:176-179, seeded at main.cpp:151 with
+   time(nullptr) * rank, 0 on one process): one genrand_real1 per chain head
+   inside a core's cone (cores in order, chains in order; :140-161), then per
+   target one genrand_real1 for the tangent angle (:167) and two
+   genrand_real3 per velocity component of each of its four beads (:18-20,
+   the logarithm's first).  Calling it again restarts the stream from seed.
+   enable = 0 returns to the keyed draws (the default).  The stream's state
+   is not part of a checkpoint.  Single-system contexts. */
+int mdpd_set_chem_mt(mdpd_ctx* ctx, uint32_t seed, int32_t enable);
+/* outputs of the MT19937 stream consumed since mdpd_set_chem_mt (0 when
+   the keyed draws are on).  No reference counterpart. */
+int mdpd_chem_mt_draws(mdpd_ctx* ctx, uint64_t* draws);
 /* stopped: an event inside the last mdpd_run reached max_amp_num; events and
    inserted (may be NULL): in-run events and amphiphiles added since
    mdpd_set_chemistry.  No reference counterpart. */

@@ -165,11 +165,16 @@ class ForceBonded {
   std::vector<uint32_t> glob_topol_;
 };
 
-// ChemManager (chemmanager.hpp:18-281) with keyed draws; the seed is the
-// chem_seed of the system's parameters.
+// ChemManager (chemmanager.hpp:18-281).  ChemManager(cp): keyed draws on
+// the chem_seed of the system's parameters.  ChemManager(cp, sys, seed): the
+// reference's draws, one MT19937 stream started by init_genrand(seed) as
+// ChemManager(seed) does (chemmanager.hpp:176-179).
 class ChemManager {
  public:
   explicit ChemManager(const mdpd_chem_params& cp) : cp_(cp) {}
+  ChemManager(const mdpd_chem_params& cp, ParticleSystem& s, uint32_t seed) : cp_(cp) {
+    Check(mdpd_set_chem_mt(s.ctx(), seed, 1), s.ctx(), "mdpd_set_chem_mt");
+  }
   // false once amp_num has reached max_amp_num (the reference's stop signal)
   bool RandomChemEvent(ParticleSystem& s) {
     Clock::Push(s);

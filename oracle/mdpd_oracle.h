@@ -31,8 +31,10 @@
  *                   and by matching the keyed mode wherever the draws do
  *                   not decide); on one process the reference seeds the
  *                   stream with time(nullptr) * rank = 0 (main.cpp:150),
- *                   which seed 0 replays.  The keyed draws themselves:
- *                   "parity unpinned" against the reference's MT sequence.
+ *                   which seed 0 replays; the engine's reference-MT mode
+ *                   is checked against it.  The keyed draws (the engine's
+ *                   default) are another stream by design, not a
+ *                   restatement of the reference's MT sequence.
  *   make_config     closed-form geometry of each mode (bond vectors, leaflet
  *                   heights and radii, ring counts floor(2 pi r sin / lip),
  *                   per-100 leaflet split), Box-Muller over the pinned Saru

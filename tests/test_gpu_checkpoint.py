@@ -213,3 +213,47 @@ def test_checkpoint_restores_chem_seed(engine_mod, tmp_path):
         assert np.array_equal(ga[k], gb[k]), k
     a.close()
     b.close()
+
+
+def test_checkpoint_load_after_chemistry_growth(engine_mod, tmp_path):
+    """A context whose chemistry appended beads loads an earlier, smaller
+    checkpoint and continues bit for bit as another context (also past its
+    first chemistry call) that loads the same file; the chemistry then runs
+    again from there with the same events in both."""
+    box = (8.0, 9.0, 8.0)
+    n_amp = 500
+    p, _ = configs.reference_params(box, pair_seed=5, init_amp_num=n_amp, dt=0.005)
+    s = configs.random_chains(n_amp, box, 4, n_solvent=300)
+    n = len(s["pos"])
+    kw = dict(p_thresld=0.05, eps=0.2, influ_rad=3.0, influ_hei=0.8, influ_dep=4.0, influ_grd=0.1,
+              max_amp_num=10 ** 6, core_ptcl_id=np.array([0, 40, 80], np.uint32))
+
+    def fresh():
+        e = engine_mod.Engine(p, capacity=n + 4 * 600)
+        e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+        e.set_topology(s["topo"], core_amp_id=[0, 10, 20])
+        e.set_clock(50, p.pair_seed)
+        return e
+
+    ck = tmp_path / "grow.ckp"
+    a = fresh()
+    a.run(3)
+    a.checkpoint_save(ck)
+    a.set_chemistry(beg_chem=0, **kw)
+    a.run(6)
+    assert a.observe()["n_amp"] > n_amp and a.num_beads() > n
+    a.checkpoint_load(ck)
+    assert a.observe()["n_amp"] == n_amp and a.num_beads() == n
+    b = fresh()
+    b.set_chemistry(beg_chem=10 ** 9, **kw)
+    b.checkpoint_load(ck)
+    for e in (a, b):
+        e.set_chemistry(beg_chem=56, **kw)
+        e.run(7)
+    ga, gb = a.download(), b.download()
+    assert a.observe()["n_amp"] == b.observe()["n_amp"] > n_amp
+    assert a.get_clock() == b.get_clock() == 60
+    for k in KEYS:
+        assert np.array_equal(ga[k], gb[k]), k
+    a.close()
+    b.close()

@@ -90,14 +90,16 @@ def test_host_driver_matches_oracle(engine_mod, oracle, tmp_path):
     assert [int(x) for x in open(os.path.join(d, "num_amp.txt")).read().split()] == [n_amp] * 4
 
 
-@pytest.mark.parametrize("max_extra", [1000, 1])
-def test_host_driver_chemistry_matches_oracle(engine_mod, oracle, tmp_path, max_extra):
+@pytest.mark.parametrize("max_extra,mt", [(1000, False), (1000, True), (1, False)])
+def test_host_driver_chemistry_matches_oracle(engine_mod, oracle, tmp_path, max_extra, mt):
     """mdpd_main --chem: the chemistry events run inside the driver's
     mdpd_run calls from --beg-chem on; the bead records (ids, types, chains)
     and amphiphile counts equal the oracle's step + RandomChemEvent loop, and
     with max_amp_num one above the start the run ends at the first event
     that inserts, as main.cpp:188 breaks (no observation of that step,
-    fin_config.xyz stamped with its time)."""
+    fin_config.xyz stamped with its time).  With --chem-mt the draws are the
+    reference's MT19937 stream seeded with 0, as the oracle's reference-MT
+    mode draws them."""
     box = (8.0, 9.0, 8.0)
     n_amp, n_sol, steps, beg = 500, 300, 10, 3
     s = configs.random_chains(n_amp, box, 4, n_solvent=n_sol)
@@ -110,9 +112,9 @@ def test_host_driver_chemistry_matches_oracle(engine_mod, oracle, tmp_path, max_
     rec["pos"], rec["vel"], rec["vel_buf"] = s["pos"], s["vel"], s["vel"]
     rec["id"], rec["prop"], rec["amp_id"], rec["unit"] = s["ids"], s["prop"], s["amp_id"], s["unit"]
     write_xyz(os.path.join(d, "init_config.xyz"), rec, 0)
-    print(f"[driver] {DRIVER} {d} --seed 77 --chem --beg-chem {beg}", flush=True)
-    r = subprocess.run([DRIVER, d, "--seed", "77", "--chem", "--beg-chem", str(beg)],
-                       capture_output=True, text=True, timeout=120)
+    cmd = [DRIVER, d, "--seed", "77", "--chem", "--beg-chem", str(beg)] + (["--chem-mt"] if mt else [])
+    print("[driver] " + " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     p, _ = engine_mod.read_run_param(d)
     p.pair_seed = 77
@@ -125,6 +127,8 @@ def test_host_driver_chemistry_matches_oracle(engine_mod, oracle, tmp_path, max_
     o.set(pos, vel, r0["id"], r0["prop"], r0["amp_id"], r0["unit"], vel_buf=r0["vel_buf"],
           acc=r0["acc"])
     o.set_topology(s["topo"], core_amp_id=[0])
+    if mt:
+        o.set_mt(0)
     cp = engine_mod.MdpdChemParams(kw["p_thresld"], kw["eps"], kw["influ_rad"], kw["influ_hei"],
                                    kw["influ_dep"], kw["influ_grd"], n_amp + max_extra, 1,
                                    engine_mod._u(np.zeros(1, np.uint32)))
