@@ -53,6 +53,7 @@ struct mdpd_ctx {
   mdpd::ChemState* chem_state = nullptr;  // [2]
   uint32_t* chem_counts = nullptr;        // [kMaxCore * chem_count_blocks(cap)]
   uint32_t* chem_targets = nullptr;       // [amp_cap]
+  unsigned long long* chem_masks = nullptr;  // [chem_mask_words(cap)] flags per (core, wave)
   uint32_t* run_ok_word = nullptr;        // the device's run word of grow mode
   mdpd::ChemDev chem_host[2] = {};        // host copies (sources of the uploads)
   // reference-MT mode (mdpd_set_chem_mt): chem_mt is the buffers of
@@ -93,9 +94,10 @@ inline void set_launch_bound(mdpd_ctx* c) { c->d.nl = (c->grow && c->d.n) ? c->c
 // the host's copies of the counts after device-side chemistry
 int pull_counts(mdpd_ctx* c);
 uint32_t chem_count_blocks(uint32_t cap);
+size_t chem_mask_words(uint32_t cap);
 void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
-                       uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
-                       const ChemMt& mt, hipStream_t st);
+                       uint32_t* counts, unsigned long long* masks, uint32_t* targets,
+                       uint32_t* amp_num, bool keys, const ChemMt& mt, hipStream_t st);
 }  // namespace mdpd
 
 #define HIPCHK(c, call)                                                                    \

@@ -318,6 +318,7 @@ int create_common(const mdpd_params* p, const uint32_t* nb, int n_rep, uint32_t 
     ALLOC(c->chem_state, 2);
     ALLOC(c->chem_counts, (size_t)kMaxCore * chem_count_blocks(cap));
     ALLOC(c->chem_targets, c->amp_cap);
+    ALLOC(c->chem_masks, chem_mask_words(cap));
     ALLOC(c->run_ok_word, 1);
   }
 #undef ALLOC
@@ -434,7 +435,7 @@ void enqueue_step(mdpd_ctx* c, bool virial, bool cm, bool first, bool chem) {
   // cell keys with the others' when this step's force pass binned them)
   if (chem)
     launch_chem_event(d, c->chem_dev, c->chem_state, c->chem_host[0].n_core, c->chem_counts,
-                      c->chem_targets, c->d_amp_num, !virial && !cm, c->chem_mt, st);
+                      c->chem_masks, c->chem_targets, c->d_amp_num, !virial && !cm, c->chem_mt, st);
   mark(9);
   mark(10);
 }

@@ -51,7 +51,7 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
   // head and unit-2 positions of every chain, from the current positions
   launch_chainpos_fill(d, c->st);
   launch_chem_event(d, c->chem_dev + 1, c->chem_state + 1, cp->n_core, c->chem_counts,
-                    c->chem_targets, c->d_amp_num, false, c->chem_mt, c->st);
+                    c->chem_masks, c->chem_targets, c->d_amp_num, false, c->chem_mt, c->st);
   HIPCHK(c, hipGetLastError());
   uint32_t res[5];
   HIPCHK(c, hipMemcpyAsync(res, c->chem_state + 1, sizeof(res), hipMemcpyDeviceToHost, c->st));

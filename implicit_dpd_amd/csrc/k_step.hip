@@ -970,11 +970,14 @@ void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3
 //   k_chem_find    storage slot of every core head (bead ids in cd->core_id)
 //   k_chem_core    per core: neighbour CMs, membrane normal h2t, the head
 //                  moved along h2t by the projection of its hydrophilic CM
-//   k_chem_count   per (core, block of 256 amphiphiles): how many heads lie
-//                  in the truncated cone and draw <= p_thresld
+//   k_chem_flags   one thread per amphiphile, every core in turn: is its
+//                  head in the core's truncated cone with a draw <=
+//                  p_thresld; the flags as one 64-bit mask per (core, wave),
+//                  their count per (core, block of 256)
 //   k_chem_scan    one block: exclusive offsets of those counts in (core,
 //                  amphiphile) order, the capacity check, the stop test
-//   k_chem_scatter the targets' amphiphile indices at those offsets
+//   k_chem_scatter the targets' amphiphile indices at those offsets, from
+//                  the masks
 //   k_chem_insert  four new beads per target, appended at n0 + 4t + u
 //   k_chem_commit  the new bead and chain counts (rep_off / amp_off on the
 //                  device), the stop of the run, the core slots reset
@@ -985,11 +988,12 @@ void launch_saru_test(const uint32_t* s1, const uint32_t* s2, const uint32_t* s3
 // chem_seed ^ TAG_TANG), the velocities (new bead id, time, chem_seed ^
 // TAG_VEL).  `time` is the device clock.  In the reference-MT mode (ChemMt)
 // the draws are the outputs of one MT19937 stream instead, in the
-// reference's order: k_chem_count first counts the heads inside each cone,
-// k_chem_scan turns the counts into each head's rank, k_chem_mt draws one
-// genrand_real1 per in-cone head in rank order, k_chem_count / k_chem_scatter
-// read the decisions by rank, and after the scan k_chem_mt draws the
-// targets' 25 outputs each for k_chem_insert.
+// reference's order: k_chem_flags first marks and counts the heads inside
+// each cone, k_chem_scan turns the counts into each head's rank, k_chem_mt
+// draws one genrand_real1 per in-cone head in rank order, k_chem_decide
+// turns the cone masks into target masks by those decisions, and after the
+// target scan k_chem_mt draws the targets' 25 outputs each for
+// k_chem_insert.
 constexpr int kChemBlock = 256;
 
 __global__ void __launch_bounds__(256) k_chem_find(DevState d, const ChemDev* __restrict__ cd,
@@ -1003,10 +1007,10 @@ __global__ void __launch_bounds__(256) k_chem_find(DevState d, const ChemDev* __
 }
 
 // one block of kCoreBlock threads per core: the neighbour sums of the core
-// head spread over the threads (cells of its window, or beads on the
-// all-pairs path), then a fixed-order tree reduction, so the result does not
-// depend on scheduling
-constexpr int kCoreBlock = 128;
+// head spread over the threads (P threads per cell of its window, each
+// taking every P-th bead of the cell, or beads on the all-pairs path), then
+// a fixed-order tree reduction, so the result does not depend on scheduling
+constexpr int kCoreBlock = 512;
 __global__ void __launch_bounds__(kCoreBlock) k_chem_core(DevState d, const ChemDev* __restrict__ cd,
                                                           ChemState* __restrict__ cs) {
 #pragma clang fp contract(off)
@@ -1045,11 +1049,15 @@ __global__ void __launch_bounds__(kCoreBlock) k_chem_core(DevState d, const Chem
       const int cx = (int)(cc % (uint32_t)nx), cy = (int)((cc / (uint32_t)nx) % (uint32_t)ny),
                 cz = (int)(cc / (uint32_t)(nx * ny));
       const int wx = 2 * rad[0] + 1, wy = 2 * rad[1] + 1, wz = 2 * rad[2] + 1;
-      for (int w = tid; w < wx * wy * wz; w += kCoreBlock) {
-        const int ox = w % wx - rad[0], oy = (w / wx) % wy - rad[1], oz = w / (wx * wy) - rad[2];
+      const int wn = wx * wy * wz;
+      const int P = wn >= kCoreBlock ? 1 : kCoreBlock / wn;
+      for (int w = tid; w < wn * P; w += kCoreBlock) {
+        const int cell = w % wn, k0 = w / wn;
+        const int ox = cell % wx - rad[0], oy = (cell / wx) % wy - rad[1], oz = cell / (wx * wy) - rad[2];
         const int qx = (cx + ox + nx) % nx, qy = (cy + oy + ny) % ny, qz = (cz + oz + nz) % nz;
         const uint32_t q = R.cell_off + (uint32_t)((qz * ny + qy) * nx + qx);
-        for (uint32_t t = d.cell_start[q]; t < d.cell_start[q + 1]; ++t) visit(t);
+        for (uint32_t t = d.cell_start[q] + (uint32_t)k0; t < d.cell_start[q + 1]; t += (uint32_t)P)
+          visit(t);
       }
     }
   }
@@ -1096,12 +1104,11 @@ __global__ void __launch_bounds__(kCoreBlock) k_chem_core(DevState d, const Chem
 // head of chain j inside the truncated cone of core c?
 __device__ __forceinline__ bool chem_in_cone(const DevState& d, const ChemDev* __restrict__ cd,
                                              const ChemState* __restrict__ cs, uint32_t c,
-                                             uint32_t j) {
+                                             const double4& hp) {
 #pragma clang fp contract(off)
   const Replica& R = d.reps[0];
   const double* ch = cs->core_geo[c];
   const double* h2t = ch + 3;
-  const double4 hp = d.chainpos[4 * (size_t)j];
   double c2p[3] = {hp.x - ch[0], hp.y - ch[1], hp.z - ch[2]};
   bd_min_image(R, c2p);
   const double depth = h2t[0] * c2p[0] + h2t[1] * c2p[1] + h2t[2] * c2p[2];
@@ -1125,44 +1132,65 @@ __device__ __forceinline__ uint32_t chem_block_rank(bool f, uint32_t* wcnt) {
   return r;
 }
 
-// the flag of chain j for core c, thread j % kChemBlock of block (j /
-// kChemBlock, c): kMode 0 keyed target, 1 head inside the cone, 2
-// reference-MT target (the in-cone draws decided in mt.cone_bits by rank).
-// A head outside the cone keeps rnd = 2.0, a target only if p_thresld >= 2.
-// Every thread of the block calls it.
+// one thread per amphiphile j (grid over the chain capacity), every core c
+// in turn: kMode 0 the keyed target test, 1 the cone test alone
+// (reference-MT mode).  A head outside the cone keeps rnd = 2.0, a target
+// only if p_thresld >= 2.  The flags go to bit j % 64 of
+// masks[c * words + j / 64], their count per block to counts[c * gridDim.x +
+// blockIdx.x]; the head position is read once for all cores.
 template <int kMode>
-__device__ __forceinline__ bool chem_flag(const DevState& d, const ChemDev* __restrict__ cd,
-                                          const ChemState* __restrict__ cs, const ChemMt& mt,
-                                          uint32_t c, uint32_t j, uint32_t* wcnt) {
+__global__ void __launch_bounds__(kChemBlock) k_chem_flags(DevState d, const ChemDev* __restrict__ cd,
+                                                           const ChemState* __restrict__ cs,
+                                                           uint32_t* __restrict__ counts,
+                                                           unsigned long long* __restrict__ masks) {
+  if (halted(d)) return;
+  const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x;
   const bool live = j < dev_namp(d);
-  const bool in = live && chem_in_cone(d, cd, cs, c, j);
-  if (kMode == 1) return in;
-  double rnd = 2.0;
-  if (kMode == 0) {
-    if (in) {
-      Saru s = saru_seed(j, *d.d_time, d.reps[0].chem_seed + TAG_CONE * (c + 1u));
-      rnd = saru_uni_f64(s);
+  const double4 hp = live ? d.chainpos[4 * (size_t)j] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const uint32_t nc = cd->n_core, time = *d.d_time, seed = d.reps[0].chem_seed;
+  const size_t words = (size_t)gridDim.x * (kChemBlock / 64);
+  const double p = cd->p_thresld;
+  for (uint32_t c = 0; c < nc; ++c) {
+    const bool in = live && chem_in_cone(d, cd, cs, c, hp);
+    bool f = in;
+    if (kMode == 0) {
+      double rnd = 2.0;
+      if (in) {
+        Saru q = saru_seed(j, time, seed + TAG_CONE * (c + 1u));
+        rnd = saru_uni_f64(q);
+      }
+      f = live && rnd <= p;
     }
-    return live && rnd <= cd->p_thresld;
+    const unsigned long long bal = __ballot(f);
+    if ((threadIdx.x & 63u) == 0u) masks[(size_t)c * words + (j >> 6)] = bal;
+    const int cnt = __syncthreads_count(f ? 1 : 0);
+    if (threadIdx.x == 0) counts[(size_t)c * gridDim.x + blockIdx.x] = (uint32_t)cnt;
   }
-  const uint32_t r = mt.cone_offs[(size_t)c * gridDim.x + blockIdx.x] + chem_block_rank(in, wcnt);
-  if (in) return (mt.cone_bits[r >> 5] >> (r & 31u)) & 1u;
-  return live && rnd <= cd->p_thresld;
 }
 
-template <int kMode>
-__global__ void __launch_bounds__(kChemBlock) k_chem_count(DevState d, const ChemDev* __restrict__ cd,
-                                                           const ChemState* __restrict__ cs,
-                                                           ChemMt mt, uint32_t* __restrict__ counts) {
+// reference-MT mode, grid (chain blocks, cores): the cone masks become
+// target masks in place, an in-cone head taking the decision of its rank
+// (mt.cone_offs + its rank in the block) from mt.cone_bits; the targets'
+// count per block to counts.  Each wave reads and rewrites only its own word.
+__global__ void __launch_bounds__(kChemBlock) k_chem_decide(DevState d, const ChemDev* __restrict__ cd,
+                                                            ChemMt mt, uint32_t* __restrict__ counts,
+                                                            unsigned long long* __restrict__ masks) {
   if (halted(d)) return;
   __shared__ uint32_t wcnt[kChemBlock / 64];
   const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
-  const bool f = chem_flag<kMode>(d, cd, cs, mt, c, j, wcnt);
+  const uint32_t lane = threadIdx.x & 63u;
+  unsigned long long* mw = masks + (size_t)c * gridDim.x * (kChemBlock / 64) + (j >> 6);
+  const bool in = ((*mw >> lane) & 1ull) != 0ull;
+  const bool live = j < dev_namp(d);
+  const uint32_t r = mt.cone_offs[(size_t)c * gridDim.x + blockIdx.x] + chem_block_rank(in, wcnt);
+  const bool f = in ? ((mt.cone_bits[r >> 5] >> (r & 31u)) & 1u) != 0u
+                    : (live && 2.0 <= cd->p_thresld);
+  const unsigned long long bal = __ballot(f);
+  if (lane == 0u) *mw = bal;
   const int cnt = __syncthreads_count(f ? 1 : 0);
   if (threadIdx.x == 0) counts[(size_t)c * gridDim.x + blockIdx.x] = (uint32_t)cnt;
 }
 
-// one block of 1024 threads: counts[0..m) -> exclusive offsets, in order
 // cone: the counts are in-cone heads (reference-MT mode), whose total is
 // the number of draws of the cone tests
 __global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* __restrict__ cd,
@@ -1204,17 +1232,18 @@ __global__ void __launch_bounds__(1024) k_chem_scan(DevState d, const ChemDev* _
   }
 }
 
-template <int kMode>
-__global__ void __launch_bounds__(kChemBlock) k_chem_scatter(DevState d, const ChemDev* __restrict__ cd,
-                                                             const ChemState* __restrict__ cs,
-                                                             ChemMt mt,
+// grid (chain blocks, cores): the targets' amphiphile indices at their
+// offsets, from the target masks
+__global__ void __launch_bounds__(kChemBlock) k_chem_scatter(DevState d, const ChemState* __restrict__ cs,
+                                                             const unsigned long long* __restrict__ masks,
                                                              const uint32_t* __restrict__ offs,
                                                              uint32_t* __restrict__ targets) {
   if (halted(d) || cs->nt == 0u) return;
-  __shared__ uint32_t wcnt[2][kChemBlock / 64];
+  __shared__ uint32_t wcnt[kChemBlock / 64];
   const uint32_t j = blockIdx.x * kChemBlock + threadIdx.x, c = blockIdx.y;
-  const bool f = chem_flag<kMode>(d, cd, cs, mt, c, j, wcnt[0]);
-  const uint32_t r = chem_block_rank(f, wcnt[1]);
+  const unsigned long long m = masks[(size_t)c * gridDim.x * (kChemBlock / 64) + (j >> 6)];
+  const bool f = ((m >> (threadIdx.x & 63u)) & 1ull) != 0ull;
+  const uint32_t r = chem_block_rank(f, wcnt);
   if (f) targets[offs[(size_t)c * gridDim.x + blockIdx.x] + r] = j;
 }
 
@@ -1424,8 +1453,8 @@ __global__ void k_chem_commit(DevState d, const ChemDev* __restrict__ cd,
 }
 
 void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,
-                       uint32_t* counts, uint32_t* targets, uint32_t* amp_num, bool keys,
-                       const ChemMt& mt, hipStream_t st) {
+                       uint32_t* counts, unsigned long long* masks, uint32_t* targets,
+                       uint32_t* amp_num, bool keys, const ChemMt& mt, hipStream_t st) {
   const uint32_t amp_cap = d.cap / 4u + 1u;
   const uint32_t nblk = div_up(amp_cap, (uint32_t)kChemBlock);
   const bool mtm = mt.g != nullptr;
@@ -1437,30 +1466,25 @@ void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint
       // at most one draw per (core, chain): n_core amp_cap bits
       launch_fill_u32(mt.cone_bits, (uint32_t)(((uint64_t)n_core * amp_cap + 31u) / 32u) + 2u, 0u,
                       st);
-      hipLaunchKernelGGL(k_chem_count<1>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
-                         mt, mt.cone_offs);
+      hipLaunchKernelGGL(k_chem_flags<1>, dim3(nblk), dim3(kChemBlock), 0, st, d, cd, ccs,
+                         mt.cone_offs, masks);
       hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, mt.cone_offs,
                          n_core * nblk, 1);
       hipLaunchKernelGGL(k_chem_mt, dim3(1), dim3(kMtBlock), 0, st, d, cd, ccs, mt, 0);
-      hipLaunchKernelGGL(k_chem_count<2>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
-                         mt, counts);
+      hipLaunchKernelGGL(k_chem_decide, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, mt,
+                         counts, masks);
     } else {
-      hipLaunchKernelGGL(k_chem_count<0>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd, ccs,
-                         mt, counts);
+      hipLaunchKernelGGL(k_chem_flags<0>, dim3(nblk), dim3(kChemBlock), 0, st, d, cd, ccs, counts,
+                         masks);
     }
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, counts, 1u, 0u);
   }
   const uint32_t m = n_core ? n_core * nblk : 1u;
   hipLaunchKernelGGL(k_chem_scan, dim3(1), dim3(1024), 0, st, d, cd, cs, counts, m, 0);
-  if (n_core) {
-    if (mtm)
-      hipLaunchKernelGGL(k_chem_scatter<2>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
-                         ccs, mt, (const uint32_t*)counts, targets);
-    else
-      hipLaunchKernelGGL(k_chem_scatter<0>, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, cd,
-                         ccs, mt, (const uint32_t*)counts, targets);
-  }
+  if (n_core)
+    hipLaunchKernelGGL(k_chem_scatter, dim3(nblk, n_core), dim3(kChemBlock), 0, st, d, ccs,
+                       (const unsigned long long*)masks, (const uint32_t*)counts, targets);
   if (mtm) hipLaunchKernelGGL(k_chem_mt, dim3(1), dim3(kMtBlock), 0, st, d, cd, ccs, mt, 1);
   hipLaunchKernelGGL(k_chem_insert, dim3(256), dim3(256), 0, st, d, cd, ccs,
                      (const uint32_t*)targets, (const uint32_t*)(mtm ? mt.tdraws : nullptr),
@@ -1468,6 +1492,9 @@ void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint
   hipLaunchKernelGGL(k_chem_commit, dim3(1), dim3(256), 0, st, d, cd, cs, amp_num);
 }
 uint32_t chem_count_blocks(uint32_t cap) { return div_up(cap / 4u + 1u, (uint32_t)kChemBlock); }
+size_t chem_mask_words(uint32_t cap) {
+  return (size_t)kMaxCore * chem_count_blocks(cap) * (kChemBlock / 64);
+}
 
 __global__ void k_copy_d(double* __restrict__ dst, const double* __restrict__ src, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
