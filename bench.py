@@ -396,7 +396,8 @@ def driver_bench(args, Engine):
     in 31x30, test/mpi_bonded/result_mpi/31x31/run_param.txt).  Its
     main loop replays mdpd_run between observation points and prints its
     own wall time; `value` is beads * steps / that time.  --chem adds the
-    RandomChemEvent branch from the first main-loop step on."""
+    RandomChemEvent branch from the first main-loop step on (--chem-mt: with
+    the reference's MT19937 draws)."""
     import subprocess
     from implicit_dpd_amd import configs
     from implicit_dpd_amd.fpdpd import FPDPD_DTYPE, write_xyz
@@ -417,7 +418,8 @@ def driver_bench(args, Engine):
     write_xyz(os.path.join(d, "init_config.xyz"), rec, 0)
     exe = os.path.join(ROOT, "implicit_dpd_amd", "bin", "mdpd_main")
     cmd = [exe, d, "--seed", "77", "--nlist-width", "192"] + (
-        ["--chem", "--beg-chem", "0"] if args.chem else [])
+        ["--chem", "--beg-chem", "0"] if args.chem else []) + (
+        ["--chem-mt"] if args.chem and args.chem_mt else [])
     log("[bench] " + " ".join(cmd))
     try:
         t = time.perf_counter()
@@ -441,7 +443,8 @@ def driver_bench(args, Engine):
                                "100x100x40, the reference's multibody model from run_param.txt "
                                "(chi 30, kappa 50, rho_co 25, cf_r 3, dt 0.005), step_mac 100, "
                                "pair-list width 192"
-                               + (", RandomChemEvent every step (4 cores, p_thresld 2e-3)"
+                               + (", RandomChemEvent every step (4 cores, p_thresld 2e-3%s)"
+                                  % (", the reference's MT19937 stream" if args.chem_mt else "")
                                   if args.chem else ""),
                    "process_wall_s": round(wall, 2)}}), flush=True)
 
@@ -479,6 +482,8 @@ def main():
                     help="time the drop-in program mdpd_main on the headline state instead")
     ap.add_argument("--chem", action="store_true",
                     help="with --driver: the RandomChemEvent branch on from the first step")
+    ap.add_argument("--chem-mt", action="store_true",
+                    help="with --driver --chem: the chemistry draws from the reference's MT19937 stream")
     ap.add_argument("--calib-bytes", type=float, default=0.0,
                     help="only run the copy-kernel probe of this many bytes (counter calibration)")
     args = ap.parse_args()
