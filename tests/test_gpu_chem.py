@@ -97,16 +97,25 @@ def test_run_chemistry_equals_explicit_events(engine_mod):
     b.close()
 
 
-def test_run_chemistry_stops_like_the_reference_loop(engine_mod):
+@pytest.mark.parametrize("mt", [False, True], ids=["keyed", "reference_mt"])
+def test_run_chemistry_stops_like_the_reference_loop(engine_mod, mt):
     """An event that brings amp_num to max_amp_num ends mdpd_run after its
     step (main.cpp:188 breaks before time++): the state and clock equal the
     explicit loop's at its break, and the next run is refused until the
-    chemistry is set again."""
+    chemistry is set again; with keyed draws and with the reference's
+    MT19937 stream (which then stands at the same draw in both)."""
     t0 = 300
-    p, s, a, core = _chem_system(engine_mod)
+
+    def system():
+        out = _chem_system(engine_mod)
+        if mt:
+            out[2].set_chem_mt(9)
+        return out
+
+    p, s, a, core = system()
     n_amp0 = len(s["topo"]) // 4
     # where does the explicit loop stop?
-    _, _, b, _ = _chem_system(engine_mod)
+    _, _, b, _ = system()
     b.set_clock(t0, p.pair_seed)
     counts = []
     for t in range(t0, t0 + 12):
@@ -121,7 +130,7 @@ def test_run_chemistry_stops_like_the_reference_loop(engine_mod):
     k_stop = counts.index(target)
     b.close()
     # the explicit loop with that max_amp_num
-    _, _, b, _ = _chem_system(engine_mod)
+    _, _, b, _ = system()
     b.set_clock(t0, p.pair_seed)
     for t in range(t0, t0 + 12):
         b.run(1)
@@ -140,6 +149,7 @@ def test_run_chemistry_stops_like_the_reference_loop(engine_mod):
     ga, na, ta = _state(a)
     assert na == nb and ta == tb
     _same(ga, gb)
+    assert a.chem_mt_draws() == b.chem_mt_draws()
     with pytest.raises(RuntimeError):
         a.run(1)
     # switched off, the run goes on from the stopped state
