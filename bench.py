@@ -333,8 +333,7 @@ def traj_io_bench(args, Engine):
     w = workload("membrane", args.chains)
     e = w.start(Engine, warmup=args.warmup)
     n = w.n
-    d = tempfile.mkdtemp(prefix="traj_", dir=os.path.join(ROOT, "gpurun_out") if os.path.isdir(
-        os.path.join(ROOT, "gpurun_out")) else None)
+    d = tempfile.mkdtemp(prefix="traj_")  # host scratch for the frames, removed below
     path = os.path.join(d, "traject.xyz")
     e.sync()
     walls = []
