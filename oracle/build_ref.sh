@@ -1,5 +1,5 @@
 #!/usr/bin/env bash
-# Builds the checker binaries in oracle/_ref/ from the reference's own sources
+# Builds test binaries in oracle/_ref/ from the reference's own sources
 # where they lie (only self-contained files: include/saruprng.hpp needs
 # nothing but <cmath>).  The rest of the reference path needs the FDPS
 # framework, which is an empty submodule in the reference tree, so it is not

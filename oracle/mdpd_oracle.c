@@ -1,8 +1,8 @@
 /*
  * mdpd_oracle.c -- CPU oracle of the MDPD step path (FP64, OpenMP).
  * TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load it, as the checker.  The engine never
- * links or calls it.  See mdpd_oracle.h for what pins each function.
+ * cpu_baseline leg of bench.py load it, to check the engine against.  The
+ * engine never links or calls it.  See mdpd_oracle.h for what pins each function.
  *
  * Every function restates one piece of kohnakagawa/implicit_dpd (cited as
  * file:line of that tree) with the same FP64 operation order, so that on

@@ -1,7 +1,8 @@
 /*
  * mdpd_oracle.h -- CPU oracle of the MDPD step path.  TEST INFRASTRUCTURE
  * ONLY: used by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
- * bench.py as the checker; never linked into or called by the engine.
+ * bench.py to check the engine against; never linked into or called by the
+ * engine.
  *
  * Plain C restatement, FP64, of the reference's algorithms (file:line of
  * kohnakagawa/implicit_dpd cited at each function in mdpd_oracle.c).

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker of the engine.  The engine never
+cpu_baseline leg of bench.py, to check the engine against.  The engine never
 imports it.
 """
 import ctypes as C

@@ -1,4 +1,4 @@
-// Checker-side driver: emits Saru PRNG vectors from the reference's own
+// Test-side driver: emits Saru PRNG vectors from the reference's own
 // header (include/saruprng.hpp, compiled where it lies under the reference
 // tree by oracle/build_ref.sh).  Nothing in the product links this; its
 // output is committed as tests/golden/saru_ref_vectors.txt and is used to pin

@@ -1,8 +1,8 @@
 #!/usr/bin/env bash
 # Regenerates tests/golden/saru_ref_vectors.txt from the reference's own
-# Saru header (include/saruprng.hpp), built by oracle/build_ref.sh.  Run in
-# the container that holds the reference tree; the GPU box only reads the
-# committed fixture.
+# Saru header (include/saruprng.hpp), built by oracle/build_ref.sh.  Run
+# where the reference tree is present; the GPU tests only read the committed
+# fixture.
 set -euo pipefail
 HERE=$(cd "$(dirname "$0")" && pwd)
 ROOT=$(cd "$HERE/../.." && pwd)
