@@ -488,7 +488,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(spatial sharding with halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
 
