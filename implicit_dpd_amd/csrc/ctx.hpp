@@ -93,6 +93,10 @@ int enter_grow(mdpd_ctx* c);
 inline void set_launch_bound(mdpd_ctx* c) { c->d.nl = (c->grow && c->d.n) ? c->cap : c->d.n; }
 // the host's copies of the counts after device-side chemistry
 int pull_counts(mdpd_ctx* c);
+// reference-MT chemistry: switch the mode on from a stream state; read the
+// state of the stream in use
+int chem_mt_enable(mdpd_ctx* c, const MtState& h);
+int chem_mt_state(mdpd_ctx* c, MtState* h);
 uint32_t chem_count_blocks(uint32_t cap);
 size_t chem_mask_words(uint32_t cap);
 void launch_chem_event(const DevState& d, const ChemDev* cd, ChemState* cs, uint32_t n_core,

@@ -1206,15 +1206,14 @@ int mdpd_set_chemistry(mdpd_ctx* c, const mdpd_chem_params* cp, uint32_t beg_che
   return MDPD_OK;
 }
 
-int mdpd_set_chem_mt(mdpd_ctx* c, uint32_t seed, int32_t enable) {
-  if (!c) return MDPD_ERR_ARG;
+}  // extern "C"
+
+namespace mdpd {
+// the reference-MT mode on, from the stream state h (buffers allocated on
+// the first use and kept)
+int chem_mt_enable(mdpd_ctx* c, const MtState& h) {
   if (c->n_rep != 1 || !c->chem_dev)
     return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
-  if (!enable) {
-    if (c->chem_mt.g) invalidate_graphs(c);  // the captured events draw from the stream
-    c->chem_mt = ChemMt{};
-    return MDPD_OK;
-  }
   int rc;
   ChemMt& m = c->mt_store;
   if (!m.g) {
@@ -1227,6 +1226,31 @@ int mdpd_set_chem_mt(mdpd_ctx* c, uint32_t seed, int32_t enable) {
       return rc;
     }
   }
+  HIPCHK(c, hipMemcpyAsync(m.g, &h, sizeof(MtState), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (!c->chem_mt.g) invalidate_graphs(c);  // the captured events change their kernels
+  c->chem_mt = m;
+  return MDPD_OK;
+}
+
+int chem_mt_state(mdpd_ctx* c, MtState* h) {
+  HIPCHK(c, hipMemcpyAsync(h, c->chem_mt.g, sizeof(MtState), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDPD_OK;
+}
+}  // namespace mdpd
+
+extern "C" {
+
+int mdpd_set_chem_mt(mdpd_ctx* c, uint32_t seed, int32_t enable) {
+  if (!c) return MDPD_ERR_ARG;
+  if (c->n_rep != 1 || !c->chem_dev)
+    return fail(c, MDPD_ERR_UNSUPPORTED, "chemistry needs a single-system context");
+  if (!enable) {
+    if (c->chem_mt.g) invalidate_graphs(c);  // the captured events draw from the stream
+    c->chem_mt = ChemMt{};
+    return MDPD_OK;
+  }
   // init_genrand (mt19937ar): the state ChemManager(seed) starts from
   MtState h;
   h.mt[0] = seed;
@@ -1234,11 +1258,7 @@ int mdpd_set_chem_mt(mdpd_ctx* c, uint32_t seed, int32_t enable) {
   h.mti = kMtN;
   h.pad = 0;
   h.draws = 0;
-  HIPCHK(c, hipMemcpyAsync(m.g, &h, sizeof(MtState), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  if (!c->chem_mt.g) invalidate_graphs(c);
-  c->chem_mt = m;
-  return MDPD_OK;
+  return chem_mt_enable(c, h);
 }
 
 int mdpd_chem_mt_draws(mdpd_ctx* c, uint64_t* draws) {
@@ -1246,8 +1266,8 @@ int mdpd_chem_mt_draws(mdpd_ctx* c, uint64_t* draws) {
   *draws = 0;
   if (!c->chem_mt.g) return MDPD_OK;
   MtState h;
-  HIPCHK(c, hipMemcpyAsync(&h, c->chem_mt.g, sizeof(MtState), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
+  const int rc = chem_mt_state(c, &h);
+  if (rc) return rc;
   *draws = h.draws;
   return MDPD_OK;
 }

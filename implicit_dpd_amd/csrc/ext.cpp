@@ -72,11 +72,13 @@ int mdpd_chem_event(mdpd_ctx* c, const mdpd_chem_params* cp, int* stop) {
 
 // ---- binary checkpoint (exact restart)
 // Layout, little-endian: magic "MDPDCKP1", u32 version (2), n_rep, n, n_amp,
-// time, 0; u32 rep_off[n_rep + 1], pair_seed[n_rep], chem_seed[n_rep],
+// time, flags; u32 rep_off[n_rep + 1], pair_seed[n_rep], chem_seed[n_rep],
 // amp_num[n_rep], sol_num[n_rep]; u64 fingerprint[n_rep] of each replica's
 // parameters; FP64 pos, vel, vel_buf, delta_sumr [3][n]; FP32 acc, press
 // [3][n]; u32 id, prop, amp_id, unit, orig, chain [n]; u32 topo[4 n_amp];
-// FP64 cf_bend[n_amp]; u32 amp_rep[n_amp].  Arrays are in the device's
+// FP64 cf_bend[n_amp]; u32 amp_rep[n_amp]; with flags bit 0 (the chemistry
+// draws from the reference's MT19937 stream) then u32 mt[624], mti, u64
+// outputs drawn.  Arrays are in the device's
 // storage order, so the next cell sort (ordered by previous slot) and every
 // FP32 sum after it repeat the uninterrupted run.
 namespace {
@@ -117,7 +119,7 @@ int mdpd_checkpoint_save(mdpd_ctx* c, const char* path) {
   if (!fp) return fail(c, MDPD_ERR_ARG, std::string("cannot open ") + path);
   bool ok = true;
   auto put = [&](const void* p, size_t bytes) { ok = ok && fwrite(p, 1, bytes, fp) == bytes; };
-  const uint32_t head[6] = {kCkpVersion, nr, n, na, c->time, 0u};
+  const uint32_t head[6] = {kCkpVersion, nr, n, na, c->time, c->chem_mt.g ? 1u : 0u};
   put(kCkpMagic, 8);
   put(head, sizeof(head));
   put(c->rep_off.data(), 4 * (size_t)(nr + 1));
@@ -169,6 +171,15 @@ int mdpd_checkpoint_save(mdpd_ctx* c, const char* path) {
     }
     if (!rc) rc = getu(d.amp_rep, na);
   }
+  if (!rc && c->chem_mt.g) {
+    MtState h;
+    rc = chem_mt_state(c, &h);
+    if (!rc) {
+      put(h.mt, sizeof(h.mt));
+      put(&h.mti, 4);
+      put(&h.draws, 8);
+    }
+  }
   if (fclose(fp) != 0) ok = false;
   if (rc) return rc;
   if (!ok) return fail(c, MDPD_ERR_ARG, std::string("write failed: ") + path);
@@ -190,7 +201,15 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
     fclose(fp);
     return fail(c, MDPD_ERR_PARSE, std::string("not a checkpoint of this format (version 2): ") + path);
   }
-  const uint32_t nr = head[1], n = head[2], na = head[3];
+  const uint32_t nr = head[1], n = head[2], na = head[3], flags = head[5];
+  if (flags > 1u) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_PARSE, std::string("unknown checkpoint flags: ") + path);
+  }
+  if ((flags & 1u) && (nr != 1u || !c->chem_dev)) {
+    fclose(fp);
+    return fail(c, MDPD_ERR_ARG, "checkpoint with a chemistry stream needs a single-system context");
+  }
   if ((int)nr != c->n_rep || n > c->cap || na > c->amp_cap) {
     fclose(fp);
     return fail(c, MDPD_ERR_ARG, "checkpoint does not fit this context (replicas or capacity)");
@@ -228,6 +247,12 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
     get(u32.data() + 6 * (size_t)n, 16 * (size_t)na);
     get(cf.data(), 8 * (size_t)na);
     get(u32.data() + 6 * (size_t)n + 4 * (size_t)na, 4 * (size_t)na);
+  }
+  MtState mts{};
+  if (flags & 1u) {
+    get(mts.mt, sizeof(mts.mt));
+    get(&mts.mti, 4);
+    get(&mts.draws, 8);
   }
   // nothing more may follow
   char extra;
@@ -278,6 +303,7 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
     }
     for (uint32_t a = 0; a < na && why.empty(); ++a)
       if (arep[a] >= nr || !std::isfinite(cf[a])) why = "chain replica or bending constant invalid";
+    if ((flags & 1u) && mts.mti > kMtN) why = "chemistry stream index out of range";
     if (!why.empty()) return fail(c, MDPD_ERR_PARSE, "corrupt checkpoint (" + why + "): " + path);
   }
   DevState& d = c->d;
@@ -328,6 +354,12 @@ int mdpd_checkpoint_load(mdpd_ctx* c, const char* path) {
   HIPCHK(c, hipMemcpy(d.d_time, &c->time, 4, hipMemcpyHostToDevice));
   c->drifted = false;
   c->have_cells = c->cells_fresh = c->have_density = false;
+  // the reference-MT chemistry stream where the saving context had it on
+  // (a file without one leaves the context's draw mode as it is)
+  if (flags & 1u) {
+    const int rc = chem_mt_enable(c, mts);
+    if (rc) return rc;
+  }
   return MDPD_OK;
 }
 

@@ -234,8 +234,9 @@ int mdpd_set_chemistry(mdpd_ctx* ctx, const mdpd_chem_params* cp, uint32_t beg_c
    target one genrand_real1 for the tangent angle (:167) and two
    genrand_real3 per velocity component of each of its four beads (:18-20,
    the logarithm's first).  Calling it again restarts the stream from seed.
-   enable = 0 returns to the keyed draws (the default).  The stream's state
-   is not part of a checkpoint.  Single-system contexts. */
+   enable = 0 returns to the keyed draws (the default).  A checkpoint saved
+   with the mode on holds the stream, and loading it switches the mode on at
+   that position.  Single-system contexts. */
 int mdpd_set_chem_mt(mdpd_ctx* ctx, uint32_t seed, int32_t enable);
 /* outputs of the MT19937 stream consumed since mdpd_set_chem_mt (0 when
    the keyed draws are on).  No reference counterpart. */

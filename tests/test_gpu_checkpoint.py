@@ -156,6 +156,8 @@ def test_corrupt_checkpoints_are_refused_before_any_copy(engine_mod, tmp_path):
         "amp_rep": lambda q: u32(q, o["amp_rep"] + 4 * 2, 3),
         "fingerprint": lambda q: q.__setitem__(slice(o["fprint"], o["fprint"] + 8), bytes(8)),
         "version1": lambda q: u32(q, 8, 1),
+        "flags": lambda q: u32(q, 8 + 20, 2),
+        "stream_missing": lambda q: u32(q, 8 + 20, 1),
         "extra": lambda q: q.extend(b"\0" * 4),
         "short": lambda q: q.__delitem__(slice(len(q) - 4, len(q))),
     }

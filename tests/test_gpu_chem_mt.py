@@ -146,3 +146,34 @@ def test_mt_run_equals_explicit_events(engine_mod):
     assert a.chem_mt_draws() == 0
     a.close()
     b.close()
+
+
+def test_mt_checkpoint_carries_the_stream(engine_mod, tmp_path):
+    """A checkpoint taken with the reference-MT draws on holds the stream: a
+    fresh context that loads it (its chemistry set, its draw mode not)
+    continues the run with the events inside bit for bit, at the same
+    position of the stream."""
+    t0 = 196
+    p, s, a, core = _run_system(engine_mod)
+    a.set_clock(t0, p.pair_seed)
+    a.set_chemistry(max_amp_num=10 ** 6, core_ptcl_id=core, beg_chem=t0, **KWR)
+    a.set_chem_mt(13)
+    a.run(4)
+    ck = tmp_path / "mt.ckp"
+    a.checkpoint_save(ck)
+    d0 = a.chem_mt_draws()
+    a.run(6)
+    ga = a.download()
+    _, _, b, _ = _run_system(engine_mod)
+    b.set_chemistry(max_amp_num=10 ** 6, core_ptcl_id=core, beg_chem=t0, **KWR)
+    assert b.chem_mt_draws() == 0
+    b.checkpoint_load(ck)
+    assert b.chem_mt_draws() == d0 > 0
+    b.run(6)
+    gb = b.download()
+    assert a.chem_mt_draws() == b.chem_mt_draws() > d0
+    assert a.observe()["n_amp"] == b.observe()["n_amp"]
+    for k in ("pos", "vel", "vel_buf", "acc", "delta", "id", "prop", "amp_id", "unit"):
+        assert np.array_equal(ga[k], gb[k]), k
+    a.close()
+    b.close()
