@@ -7,7 +7,9 @@
   force pass walks) and of rd equal the FP64 oracle's cell-list pair sets
   (bit-exact); densities within 2e-6 relative and forces within 1e-5 of the
   per-bead pair-magnitude sum (floored at one unit of force), against the
-  oracle on the same state.
+  oracle on the same state; the membrane also relaxed (2200 steps).
+- RandomChemEvent on the membrane (keyed and reference-MT draws) against
+  the oracle's event on the same state.
 - Groot-Warren fluid, 24,000 beads, 1000 steps: kinetic temperature within
   1.5% of kT, pressure within 1.5% of the oracle's run of the same system,
   and the excess pressure (p - rho kT) / (a rho^2) in [0.085, 0.105] (the
@@ -87,6 +89,56 @@ def test_full_size_pairs_density_force_match_oracle(engine_mod, oracle, name, st
     t = e.get_clock()
     e.close()
     _check_state(oracle, p, g, [x[0] for x in pairs], [x[1] for x in pairs], t)
+
+
+@pytest.mark.parametrize("mt", [False, True], ids=["keyed", "reference_mt"])
+def test_membrane_full_size_chem_event_matches_oracle(engine_mod, oracle, mt):
+    """RandomChemEvent on the 1,048,576-bead membrane after 20 steps (four
+    core amphiphiles, cones of radius 3 and depth -1..3, p_thresld 0.3), with
+    the keyed draws and with the reference's MT19937 stream seeded with 0:
+    the inserted chains' ids, types and counts exact, positions to 1e-12,
+    velocities to 1e-13 relative and the stream's draw count exact, against
+    the oracle on the same state."""
+    print(f"[fullsize] membrane chemistry event, mt={mt}", flush=True)
+    p, s = configs.membrane_stack()
+    n, n_amp = len(s["pos"]), len(s["topo"]) // 4
+    cap = n + 4 * 4096
+    e = engine_mod.Engine(p, capacity=cap)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(0, p.pair_seed)
+    e.run(20)
+    g0 = e.download()
+    t = e.get_clock()
+    if mt:
+        e.set_chem_mt(0)
+    e.rebuild()
+    e.density()
+    kw = dict(p_thresld=0.3, eps=0.2, influ_rad=3.0, influ_hei=1.0, influ_dep=3.0, influ_grd=0.0)
+    core = np.array([0, 4, 8, 12], np.uint32)
+    stop = e.chem_event(core_ptcl_id=core, max_amp_num=10 ** 9, **kw)
+    g = e.download()
+    draws = e.chem_mt_draws()
+    e.close()
+    o = oracle.System(p, cap, n_amp + 4096)
+    o.set(g0["pos"], g0["vel"], g0["id"], g0["prop"], g0["amp_id"], g0["unit"])
+    o.set_topology(s["topo"])
+    o.set_clock(t)
+    if mt:
+        o.set_mt(0)
+    cp = engine_mod.MdpdChemParams(kw["p_thresld"], kw["eps"], kw["influ_rad"], kw["influ_hei"],
+                                   kw["influ_dep"], kw["influ_grd"], 10 ** 9, len(core),
+                                   engine_mod._u(core))
+    k, ostop = o.chem_event(cp)
+    r = o.get()
+    assert k > 10 and stop == ostop == 0, k
+    assert len(g["pos"]) == len(r["pos"]) == n + 4 * k
+    for key in ("id", "prop", "amp_id", "unit"):
+        assert np.array_equal(g[key], r[key]), key
+    assert np.abs(g["pos"] - r["pos"]).max() < 1e-12
+    assert np.array_equal(g["vel"][:n], r["vel"][:n])
+    assert np.allclose(g["vel"][n:], r["vel"][n:], rtol=1e-13, atol=1e-14)
+    assert draws == (o.mt_draws() if mt else 0)
 
 
 def test_membrane_full_size_observables(engine_mod, oracle):
