@@ -177,3 +177,45 @@ def test_mt_checkpoint_carries_the_stream(engine_mod, tmp_path):
         assert np.array_equal(ga[k], gb[k]), k
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("mt", [False, True], ids=["keyed", "reference_mt"])
+def test_p_thresld_two_makes_every_head_a_target(engine_mod, oracle, mt):
+    """A head outside every cone keeps rnd = 2.0 (chemmanager.hpp:143), so
+    with p_thresld >= 2 every head of every core's loop is a target, in cone
+    or not: n_core * n_amp insertions, in (core, chain) order, the in-cone
+    heads still drawing their (now irrelevant) numbers in the MT mode."""
+    box = (6.0, 6.0, 6.0)
+    n_amp = 100
+    p, _ = configs.reference_params(box, pair_seed=5, init_amp_num=n_amp, dt=0.005)
+    s = configs.random_chains(n_amp, box, 8, n_solvent=100)
+    n = len(s["pos"])
+    cap = n + 4 * 2 * n_amp
+    e = engine_mod.Engine(p, capacity=cap)
+    e.upload(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    e.set_topology(s["topo"])
+    e.set_clock(40, p.pair_seed)
+    o = oracle.System(p, cap, 3 * n_amp)
+    o.set(s["pos"], s["vel"], s["ids"], s["prop"], s["amp_id"], s["unit"])
+    o.set_topology(s["topo"])
+    o.set_clock(40)
+    if mt:
+        e.set_chem_mt(4)
+        o.set_mt(4)
+    kw = dict(KW1, p_thresld=2.0)
+    ids, cp = _cp(engine_mod, kw, [3, 7], 10 ** 6)
+    e.rebuild()
+    e.density()
+    e.chem_event(core_ptcl_id=ids, max_amp_num=10 ** 6, **kw)
+    k, _ = o.chem_event(cp)
+    assert k == 2 * n_amp
+    g, r = e.download(), o.get()
+    assert len(g["pos"]) == len(r["pos"]) == n + 4 * k
+    for key in ("id", "prop", "amp_id", "unit"):
+        assert np.array_equal(g[key], r[key]), key
+    assert np.abs(g["pos"] - r["pos"]).max() < 1e-12
+    assert np.allclose(g["vel"][n:], r["vel"][n:], rtol=1e-13, atol=1e-14)
+    assert e.chem_mt_draws() == (o.mt_draws() if mt else 0)
+    if mt:
+        assert o.mt_draws() > 25 * k   # the in-cone heads' draws came first
+    e.close()
