@@ -174,3 +174,32 @@ def test_chemistry_beyond_capacity_is_refused(engine_mod):
         e.run(3)
     assert e.num_beads() == n0 and e.chem_status()[0]
     e.close()
+
+
+def test_pair_sets_after_insertions_match_brute_force(engine_mod, oracle):
+    """The neighbour set after particles are appended (the reference's
+    test/increase_n, increase_test.cpp:120-190: +20 particles per iteration
+    over 20 iterations, the list compared with a brute-force one): here
+    twenty steps with RandomChemEvent inside, each appending chains; after
+    every one the cell list of the grown system gives the FP64 brute-force
+    pair sets at 1 and rd, bit for bit."""
+    p, s, e, core = _chem_system(engine_mod)
+    n0 = len(s["pos"])
+    e.set_clock(500, p.pair_seed)
+    # chains inserted next to their templates crowd the cones: room for
+    # the denser neighbourhoods (the overflow check stops the run otherwise)
+    e.set_tuning(nlist_width=192)
+    e.set_chemistry(max_amp_num=10 ** 6, core_ptcl_id=core, beg_chem=0, **KW)
+    sizes = []
+    for it in range(20):
+        e.run(1)
+        e.rebuild()
+        g = e.download()
+        sizes.append(len(g["pos"]))
+        for cut in sorted({1.0, p.rd}):
+            gi, gj = e.pairs(cut)
+            oi, oj = oracle.pairs(p, g["pos"], cut, brute=True)
+            assert len(gi) == len(oi), (it, cut)
+            assert np.array_equal(gi, oi) and np.array_equal(gj, oj), (it, cut)
+    assert sizes[-1] > n0 and len(set(sizes)) > 5, sizes
+    e.close()
