@@ -932,7 +932,8 @@ int oracle_sys_chem_event(oracle_sys* s, const mdpd_chem_params* cp, int32_t* st
         if (s->mt_on) {
           /* GenThermalVeloc (:18-20), T = 1: sqrt(-2 log u) cos(2 pi u') with
              u, u' two genrand_real3 draws; C++ leaves the order of the two
-             calls in that expression unspecified, here the log's comes first */
+             calls in that expression open, and g++ (the reference's
+             compiler, at its Makefile flags) takes the log's first, as here */
           const double u = o_mt_real3(&s->mt), u2 = o_mt_real3(&s->mt);
           s->mt_draws += 2;
           s->vel[3 * i + a] = sqrt(-2.0 * log(u)) * cos(2.0 * M_PI * u2);

@@ -392,3 +392,38 @@ def test_reference_mt_mode_threshold_statistics(oracle):
         hits += k
     # 40 events x 5 candidates at p = 0.3: mean 60, sd 6.5
     assert 30 < hits < 90
+
+
+# A thermal velocity component is sqrt(-2 T log(u)) cos(2 pi u') with u, u'
+# two genrand_real3 calls inside one expression (chemmanager.hpp:18-20),
+# whose evaluation order C++ leaves open.  The oracle's MT mode takes the
+# logarithm's draw first; this probe, an expression of that shape compiled
+# with the reference's Makefile compiler and flags (g++ -O3 -std=c++11
+# -ffast-math -funroll-loops), shows that g++ does the same.
+_ORDER_PROBE = r"""
+#include <cmath>
+#include <cstdio>
+static int n_calls = 0;
+__attribute__((noinline)) double draw() { return 0.3 + 0.3 * n_calls++; }
+static inline double thermal(double kt) {
+  return std::sqrt(-2.0 * kt * std::log(draw())) * std::cos(2.0 * M_PI * draw());
+}
+int main() { std::printf("%.17g\n", thermal(1.0)); return 0; }
+"""
+
+
+@pytest.mark.parametrize("flags", [["-O3", "-std=c++11", "-ffast-math", "-funroll-loops"],
+                                   ["-O0", "-std=c++11"]])
+def test_gcc_draws_the_logarithm_first(tmp_path, flags):
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "order.cpp"
+    src.write_text(_ORDER_PROBE)
+    exe = tmp_path / "order"
+    subprocess.run(["g++"] + flags + [str(src), "-o", str(exe)], check=True)
+    v = float(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    log_first = math.sqrt(-2.0 * math.log(0.3)) * math.cos(2.0 * math.pi * 0.6)
+    cos_first = math.sqrt(-2.0 * math.log(0.6)) * math.cos(2.0 * math.pi * 0.3)
+    assert abs(v - log_first) < 1e-12 and abs(v - cos_first) > 0.5, (v, log_first, cos_first)
