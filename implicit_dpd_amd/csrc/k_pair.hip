@@ -179,8 +179,11 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
 // pair-list write run there with most lanes busy, rather than inside the
 // scan with only the lanes whose candidate is in range (about one in six).
 // Each lane still meets its in-range candidates in scan order, so the FP32
-// sums and the pair list are those of a plain per-candidate loop.
-constexpr uint32_t kDensQ = 8;
+// sums and the pair list are those of a plain per-candidate loop.  Ten
+// entries of 8 bytes per lane are 5 KB of LDS per wave, which still lets
+// 8 waves per SIMD in (more than the 70 VGPRs allow): deeper queues drain
+// less often, and at 12 the LDS would cost a wave.
+constexpr uint32_t kDensQ = 10;
 
 // The queued r^2 word.  A kept candidate has r^2 < 1 + kGuard < 2 (rd <= 1),
 // so the float's two top bits (sign and the exponent's top bit) are zero;
