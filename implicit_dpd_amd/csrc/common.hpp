@@ -84,6 +84,12 @@ __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a +
 // slower at 24k and 198k).
 constexpr uint32_t kStreamMin = 1u << 19;
 
+// a 16-byte streaming (non-temporal) load: one global_load_dwordx4 ... nt
+__device__ __forceinline__ uint4 ld_stream_u4(const uint4* p) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 template <bool ST, class T>
 __device__ __forceinline__ T ld_hint(const T* p) {
   if constexpr (ST) return __builtin_nontemporal_load(p);
