@@ -179,11 +179,12 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
 // pair-list write run there with most lanes busy, rather than inside the
 // scan with only the lanes whose candidate is in range (about one in six).
 // Each lane still meets its in-range candidates in scan order, so the FP32
-// sums and the pair list are those of a plain per-candidate loop.  Ten
-// entries of 8 bytes per lane are 5 KB of LDS per wave, which still lets
-// 8 waves per SIMD in (more than the 70 VGPRs allow): deeper queues drain
-// less often, and at 12 the LDS would cost a wave.
+// sums and the pair list are those of a plain per-candidate loop.  The
+// scan takes four candidates per iteration (four gathers in flight per
+// wave; 80 VGPRs, 6 waves per SIMD).  Ten entries of 8 bytes per lane are
+// 5 KB of LDS per wave, which costs no wave.
 constexpr uint32_t kDensQ = 10;
+constexpr uint32_t kScanStep = 4;  // candidates per scan iteration
 
 // The queued r^2 word.  A kept candidate has r^2 < 1 + kGuard < 2 (rd <= 1),
 // so the float's two top bits (sign and the exponent's top bit) are zero;
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
   // [qi, qe) is in use, and past qhi fewer than two places are free
   constexpr uint32_t qs = BS * (uint32_t)sizeof(uint2);
   const uint32_t qi = threadIdx.x * (uint32_t)sizeof(uint2);
-  const uint32_t qhi = qi + (kDensQ - 2u) * qs;
+  const uint32_t qhi = qi + (kDensQ - kScanStep) * qs;
   uint32_t qe = qi;
   char* const qbase = reinterpret_cast<char*>(dens_queue);
   float rho[ND];
@@ -297,25 +298,27 @@ __global__ void __launch_bounds__(BS) k_density_q(DevState d) {
           else { ta = S[0]; tb = S[1]; fx = 2u; }
         }
         const uint32_t Xp = Xi + fix_wrap(fx, Wx), C = cyz | (fx << kCodeShift);
-        // two candidates per iteration, two gathers in flight per wave; the
-        // second of an odd run's last pair is a repeated load, not queued.
-        // The queue keeps room for both (drained when fewer than two are free).
-        for (uint32_t t = ta; t < tb; t += 2u) {
-          const uint32_t u = min(t + 1u, tb - 1u);
-          const float4 pa = ld_slot(d.spos, t), pb = ld_slot(d.spos, u);
-          {
-            const float dx = fix_dx(Xp, __float_as_uint(pa.x), hs), dy = by - pa.y, dz = bz - pa.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            // written always (the queue has room for two more), kept only
-            // in range: no exec-mask branch per candidate
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(t | C, queue_r2<ND>(r2, pa.w));
-            qe += r2 < cut_hi ? qs : 0u;
+        // kScanStep candidates per iteration, as many gathers in flight per
+        // wave; past the run's end the slot repeats the run's last one and
+        // is not queued.  The queue keeps room for kScanStep more (drained
+        // when fewer places are free).
+        for (uint32_t t = ta; t < tb; t += kScanStep) {
+          uint32_t tq[kScanStep];
+          float4 pq[kScanStep];
+#pragma unroll
+          for (uint32_t q = 0; q < kScanStep; ++q) {
+            tq[q] = min(t + q, tb - 1u);
+            pq[q] = ld_slot(d.spos, tq[q]);
           }
-          {
-            const float dx = fix_dx(Xp, __float_as_uint(pb.x), hs), dy = by - pb.y, dz = bz - pb.z;
+#pragma unroll
+          for (uint32_t q = 0; q < kScanStep; ++q) {
+            const float dx = fix_dx(Xp, __float_as_uint(pq[q].x), hs), dy = by - pq[q].y,
+                        dz = bz - pq[q].z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(u | C, queue_r2<ND>(r2, pb.w));
-            qe += (r2 < cut_hi && u != t) ? qs : 0u;
+            // written always (the queue has room), kept only in range and
+            // inside the run: no exec-mask branch per candidate
+            *reinterpret_cast<uint2*>(qbase + qe) = make_uint2(tq[q] | C, queue_r2<ND>(r2, pq[q].w));
+            qe += (r2 < cut_hi && t + q < tb) ? qs : 0u;
           }
           if (__ballot(qe > qhi) != 0ull) drain();
         }
