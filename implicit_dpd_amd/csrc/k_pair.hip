@@ -601,11 +601,10 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   float band_e = 0.f, fsum = 0.f;
   double pid[3];  // the bead's FP64 master position (exact walk only)
   // exact: 1 - r of band pairs from the FP64 master positions
-  auto pair = [&](uint32_t e, bool exact) {
+  // pj, vj: the neighbour's record sb[e & kIndexMask]
+  auto pair = [&](uint32_t e, const float4 pj, const float4 vj, bool exact) {
     const uint32_t t = e & kIndexMask;
     const uint32_t code = e >> kCodeShift;
-    const float4 pj = d.sb[t].pos;
-    const float4 vj = d.sb[t].vel;
     float dx, dy, dz;
     if (!MULTI) {
       const float4 o = otab[code];
@@ -709,15 +708,26 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
   // in large systems streaming loads keep its 4 B per pair from displacing
   // neighbour records in L2 (1M beads: force 157.5 -> 151.0 us; the 198k
   // slab and the 24k fluid, whose data fit in the caches, are better
-  // without)
+  // without).  Each chunk's four records are gathered before its first pair
+  // (four gathers in flight per lane; the unused entries of a list's last
+  // chunk are 0, slot 0, so every load is in bounds) and the next chunk of
+  // the list is loaded meanwhile: 95 VGPRs and 5 waves per SIMD instead of
+  // 70 and 7, the pass waits less on its gathers.
   const bool stream = d.nl >= kStreamMin;
-  for (uint32_t k = 0; k < cnt; k += 4) {
+  auto ldl = [&](uint32_t k) -> uint4 {
     const uint4* q = lst + (size_t)(k >> 2) * 64u;
-    const uint4 ch = stream ? ld_stream_u4(q) : *q;
-    pair(ch.x, false);
-    if (k + 1 < cnt) pair(ch.y, false);
-    if (k + 2 < cnt) pair(ch.z, false);
-    if (k + 3 < cnt) pair(ch.w, false);
+    return stream ? ld_stream_u4(q) : *q;
+  };
+  uint4 nxt = cnt ? ldl(0) : make_uint4(0u, 0u, 0u, 0u);
+  for (uint32_t k = 0; k < cnt; k += 4) {
+    const uint4 ch = nxt;
+    if (k + 4 < cnt) nxt = ldl(k + 4);
+    const SBead b0 = d.sb[ch.x & kIndexMask], b1 = d.sb[ch.y & kIndexMask],
+                b2 = d.sb[ch.z & kIndexMask], b3 = d.sb[ch.w & kIndexMask];
+    pair(ch.x, b0.pos, b0.vel, false);
+    if (k + 1 < cnt) pair(ch.y, b1.pos, b1.vel, false);
+    if (k + 2 < cnt) pair(ch.z, b2.pos, b2.vel, false);
+    if (k + 3 < cnt) pair(ch.w, b3.pos, b3.vel, false);
   }
   if (kDrEst * band_e > 0.5e-5f * fmaxf(fsum, 1.0e-3f)) {
     // rare: the same walk with the band's 1 - r from FP64
@@ -726,10 +736,11 @@ __global__ void __launch_bounds__(256) k_force(DevState d, double* __restrict__ 
     fx = fy = fz = px = py = pz = 0.f;
     for (uint32_t k = 0; k < cnt; k += 4) {
       const uint4 ch = lst[(size_t)(k >> 2) * 64u];
-      pair(ch.x, true);
-      if (k + 1 < cnt) pair(ch.y, true);
-      if (k + 2 < cnt) pair(ch.z, true);
-      if (k + 3 < cnt) pair(ch.w, true);
+      const uint32_t e4[4] = {ch.x, ch.y, ch.z, ch.w};
+      for (uint32_t q = 0; q < 4 && k + q < cnt; ++q) {
+        const SBead& b = d.sb[e4[q] & kIndexMask];
+        pair(e4[q], b.pos, b.vel, true);
+      }
     }
   }
   if (VIRIAL) {
