@@ -174,8 +174,9 @@ __global__ void __launch_bounds__(256) k_skin_build(DevState d) {
 // Density pass, queued form (the every-step pass).  The candidate scan
 // only forms the FP32 r^2; a candidate inside cut_hi goes into the lane's
 // queue in LDS as (entry, r^2), kDensQ deep.  Once some lane of the wave has
-// fewer than two free places (the scan takes two candidates per iteration),
-// and after the scan, every active lane drains its own queue in order: the guard-band decisions, the density weight and the
+// fewer than kScanStep free places (the candidates of one scan iteration),
+// and after the scan, every active lane drains its own queue in order: the
+// guard-band decisions, the density weight and the
 // pair-list write run there with most lanes busy, rather than inside the
 // scan with only the lanes whose candidate is in range (about one in six).
 // Each lane still meets its in-range candidates in scan order, so the FP32
